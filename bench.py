@@ -1,0 +1,155 @@
+"""Benchmark of the topeFFT transform path on one MI355X.
+
+Workload (one "step"): a forward 2D C2C FP64 transform of a 4096 x 4096
+complex128 array (256 MiB) already resident in HBM, executed through the
+C-ABI's device-resident entry point (tope2DExecDevice).  Input: seeded
+uniform [-1,1) re/im plus 8 planted unit exponentials.
+
+value = algorithmic GB/s = 2 * 16 B * N * P / t, P = 2 (one HBM read and
+one write per axis), N = 4096^2 -> 1 GiB per transform.  Also printed:
+transforms/s, the same figures for the 3D 256^3 transform, and for the
+dominant kernel its HIP-event duration against the 8 TB/s HBM peak.
+
+cpu_baseline: the oracle's CPU restatement of the reference algorithm
+(radix-8 DIT + bit-reversal swap + one full pass per stage, OpenMP) on the
+same 2D input, timed on the host over a bounded number of transforms.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "tope-fft_amd", "python"))
+
+HBM_PEAK_GBS = 8000.0
+FP64_VECTOR_PEAK_TFLOPS = 78.6
+
+
+def gbs(nbytes, seconds):
+    return nbytes / seconds / 1e9
+
+
+def measure(fw, plan, steps, warmup):
+    import topefft
+    for _ in range(warmup):
+        plan.exec_device(topefft.FORWARD)
+    fw.synchronize()
+    plan.pass_timing(True)
+    fw.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.exec_device(topefft.FORWARD)
+    fw.synchronize()
+    t1 = time.perf_counter()
+    pass_ms = plan.pass_times()
+    plan.pass_timing(False)
+    return (t1 - t0) / steps, pass_ms
+
+
+def cpu_baseline(x, dims, budget_s=12.0):
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle_py
+    oracle_py.load()
+    y = np.array(x, copy=True)
+    oracle_py.refcpu(y, dims, oracle_py.FORWARD, inplace=True)  # warm-up
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < 5 and (time.perf_counter() - t_start) < budget_s:
+        y[:] = x
+        t0 = time.perf_counter()
+        oracle_py.refcpu(y, dims, oracle_py.FORWARD, inplace=True)
+        times.append(time.perf_counter() - t0)
+    return float(np.median(times)), len(times), oracle_py.threads()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(each box has one MI355X); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import topefft
+    import workloads
+
+    fw = topefft.Framework()
+
+    # ---- 2D 4096^2 FP64 (the headline workload) --------------------------
+    dims2 = (4096, 4096)
+    n2 = dims2[0] * dims2[1]
+    x2, _ = workloads.planted_input(workloads.SEED_2D, (4096, 4096, 1))
+    p2 = topefft.Plan(fw, dims2, x2)
+    t2, pass2 = measure(fw, p2, args.steps, args.warmup)
+    names2 = p2.pass_names()
+    alg2 = 2 * 16 * n2 * 2
+    value = gbs(alg2, t2)
+    dom = int(np.argmax(pass2))
+    dom_bytes = 2 * 16 * n2  # one read + one write of the array per pass
+    achieved = gbs(dom_bytes, pass2[dom] * 1e-3)
+
+    # ---- 3D 256^3 FP64 (same recipe, beside it) ---------------------------
+    dims3 = (256, 256, 256)
+    n3 = 256 ** 3
+    x3, _ = workloads.planted_input(workloads.SEED_3D, dims3)
+    p3 = topefft.Plan(fw, dims3, x3)
+    t3, pass3 = measure(fw, p3, max(5, args.steps // 2), args.warmup)
+    alg3 = 2 * 16 * n3 * 3
+    p3.destroy()
+    del x3
+
+    cpu = None
+    if not args.no_cpu:
+        ct, reps, cores = cpu_baseline(x2, dims2)
+        cpu = {"value": round(gbs(alg2, ct), 3), "unit": "GB/s", "cores": cores, "kind": "port",
+               "sample": f"median of {reps} forward 2D 4096x4096 C2C FP64 transforms by the oracle's "
+                         f"CPU restatement of the reference algorithm (radix-8 DIT, bit-reversal swap, "
+                         f"one pass per stage), same input, {cores} OpenMP threads, "
+                         f"{ct * 1e3:.1f} ms per transform"}
+
+    flops2 = 5.0 * n2 * np.log2(n2)
+    line = {
+        "metric": "GB/s & transforms/s, FP64 vs mixed-prec MFMA at rel-L2<=1e-6, 2D 4096^2 C2C",
+        "value": round(value, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t2 * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded uniform [-1,1) re/im + 8 planted unit exponentials",
+        "config": {"workload": "2D C2C FP64 4096x4096 complex128 forward, device-resident",
+                   "transforms_per_s": round(1.0 / t2, 2),
+                   "fp64_tflops_5nlogn": round(flops2 / t2 / 1e12, 3),
+                   "algorithmic_bytes_per_transform": alg2,
+                   "passes": {n: round(float(m), 4) for n, m in zip(names2, pass2)},
+                   "3d_256cubed": {"gbs": round(gbs(alg3, t3), 2), "ms": round(t3 * 1e3, 4),
+                                   "transforms_per_s": round(1.0 / t3, 2),
+                                   "pass_ms": [round(float(m), 4) for m in pass3]}},
+        "roofline": {"bound": "hbm", "kernel": names2[dom], "achieved": round(achieved, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None,
+                     "per_launch_algorithmic_bytes": dom_bytes,
+                     "avg_launch_ms": round(float(pass2[dom]), 5)},
+        "cpu_baseline": cpu,
+    }
+    p2.destroy()
+    fw.destroy()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

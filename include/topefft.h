@@ -1,0 +1,198 @@
+/*
+ * topefft.h -- drop-in C interface of the MI355X (gfx950) topeFFT library.
+ *
+ * Source-compatible superset of the reference header src/topefft.h:
+ * the same struct tags, public field names, constants and the $CHECKERROR
+ * macro, with OpenCL handles replaced by opaque pointers and <stdint.h>
+ * types, and with the prototypes the reference header never declared
+ * (callers of the reference relied on implicit declarations).
+ *
+ * Data layout (unchanged): interleaved complex128, x fastest,
+ *     d[2*((z*y_len + j)*x_len + i)] = re,  d[... + 1] = im.
+ * Semantics (unchanged): PlanInit copies d to the device; Exec transforms
+ * the device-resident copy in place and writes the result into d; FORWARD
+ * uses W = e^{-2 pi i / N} unnormalised, INVERSE the conjugate and divides
+ * by the total element count.
+ *
+ * Documented deviations from the reference (bugs not reproduced):
+ *   - tope1DExec returns the transformed data (the reference reads an
+ *     unallocated scratch buffer on its radix path, src/fft1d.c:140-142).
+ *   - 1D sizes that are not powers of two run a GPU DFT in both directions
+ *     (the reference's DFT kernel ignores `dir`, src/kernels1D.cl:549-573).
+ *   - the n == 2 1D inverse divides both real and imaginary parts by 2
+ *     (src/fft1d.c:23-26 divides only the real parts).
+ *   - tope*Destroy frees the plan; the framework (stream) is released when
+ *     its last plan is destroyed or by topeFFTDestroy (the reference
+ *     releases the whole OpenCL context on every Destroy, src/fft2d.c:453-462).
+ *   - errors set f->error and return instead of assert/exit
+ *     (src/fft2d.c:355-362, src/util.c:149-151).
+ */
+#ifndef TOPEFFT_H
+#define TOPEFFT_H
+
+#include <assert.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* reference src/topefft.h:4-11 */
+#define MAX_BINARY_SIZE (0x100000)
+#define MAX_SOURCE_SIZE (0x100000)
+#define TOPE_SUCCESS 0
+#define $CHECKERROR assert(f->error == TOPE_SUCCESS);
+
+#define C2C 1
+#define R2C 0
+#define FORWARD 1
+#define INVERSE 0
+
+/* status codes stored in f->error (HIP runtime errors are stored as their
+ * positive hipError_t value) */
+#define TOPE_ERROR_INVALID_SIZE (-1001)
+#define TOPE_ERROR_INVALID_ARG (-1002)
+#define TOPE_ERROR_NO_DEVICE (-1003)
+#define TOPE_ERROR_NOT_INITIALIZED (-1004)
+#define TOPE_ERROR_NO_KERNEL (-1005)
+#define TOPE_ERROR_UNSUPPORTED (-1006)
+
+typedef uint64_t tope_ulong; /* was cl_ulong */
+
+/* Plan structs: reference src/topefft.h:13-81.  `data` is the device buffer
+ * holding the transform (interleaved complex128); `impl` is the opaque
+ * schedule.  Counters are nanoseconds, filled from HIP events:
+ * totalMemory = host<->device copies, totalPreKernel = plan-time twiddle
+ * generation, totalKernel = transform kernels. */
+struct topePlan1D {
+    int x;           /* length */
+    int log;         /* reference stage count of the radix rule */
+    int type;        /* C2C (R2C is stored, as in the reference, and runs C2C) */
+    int radix;       /* reference radix rule: 8, 4, 2, or -1 (DFT path) */
+    void *data;      /* device buffer (complex128, x elements) */
+    void *scratch;   /* device scratch for multi-pass schedules, or NULL */
+    void *twiddle;   /* device twiddle tables */
+    size_t dataSize; /* bytes */
+    tope_ulong totalMemory;
+    tope_ulong totalPreKernel;
+    tope_ulong totalKernel;
+    unsigned int dim;
+    void *impl;
+};
+
+struct topePlan2D {
+    int x, y;
+    int logX, logY;
+    int type;
+    int radX, radY;
+    void *data;
+    void *scratch;
+    void *twdX, *twdY;
+    size_t dataSize;
+    tope_ulong totalMemory;
+    tope_ulong totalPreKernel;
+    tope_ulong totalKernel;
+    unsigned int dim;
+    void *impl;
+};
+
+struct topePlan3D {
+    int x, y, z;
+    int logX, logY, logZ;
+    int type;
+    int radX, radY, radZ;
+    void *data;
+    void *scratch;
+    void *twdX, *twdY, *twdZ;
+    size_t dataSize;
+    tope_ulong totalMemory;
+    tope_ulong totalPreKernel;
+    tope_ulong totalKernel;
+    unsigned int dim;
+    void *impl;
+};
+
+/* Framework state: reference src/topefft.h:83-95. */
+struct topeFFT {
+    int error;                      /* last status, TOPE_SUCCESS == 0 */
+    void *event;                    /* event pair of the last exec (profileThis) */
+    int device;                     /* HIP device ordinal */
+    void *stream;                   /* hipStream_t all plans enqueue on */
+    void *impl;                     /* opaque framework state */
+    unsigned int ret_num_platforms; /* 1 */
+    unsigned int ret_num_devices;   /* HIP devices visible */
+};
+
+/* ---- reference entry points ------------------------------------------ */
+
+/* src/util.c:21 */
+void topeFFTInit(struct topeFFT *f);
+
+/* src/fft1d.c:303 / :12 / :372 */
+void tope1DPlanInit(struct topeFFT *f, struct topePlan1D *t, int x, int type, double *d);
+void tope1DExec(struct topeFFT *f, struct topePlan1D *t, double *d, int dir);
+void tope1DDestroy(struct topeFFT *f, struct topePlan1D *t);
+
+/* src/fft2d.c:319 / :136 / :442 */
+void tope2DPlanInit(struct topeFFT *f, struct topePlan2D *t, int x, int y, int type, double *d);
+void tope2DExec(struct topeFFT *f, struct topePlan2D *t, double *d, int dir);
+void tope2DDestroy(struct topeFFT *f, struct topePlan2D *t);
+
+/* src/fft3d.c:474 / :216 / :628 */
+void tope3DPlanInit(struct topeFFT *f, struct topePlan3D *t, int x, int y, int z, int type, double *d);
+void tope3DExec(struct topeFFT *f, struct topePlan3D *t, double *d, int dir);
+void tope3DDestroy(struct topeFFT *f, struct topePlan3D *t);
+
+/* src/util.c:12 -- END-START nanoseconds of an event; pass f->event for the
+ * device time of the last transform (waits for it to finish). */
+tope_ulong profileThis(void *event);
+
+/* ---- additions (no reference counterpart) ------------------------------ */
+
+/* Release the framework's stream and state (the reference folds this into
+ * every tope*Destroy). */
+void topeFFTDestroy(struct topeFFT *f);
+
+/* Device-resident execution: transform t->data in place on f->stream and
+ * return without waiting and without any host copy. */
+void tope1DExecDevice(struct topeFFT *f, struct topePlan1D *t, int dir);
+void tope2DExecDevice(struct topeFFT *f, struct topePlan2D *t, int dir);
+void tope3DExecDevice(struct topeFFT *f, struct topePlan3D *t, int dir);
+
+/* Copy between the host and a plan's device buffer (blocking). */
+void topePlanUpload(struct topeFFT *f, void *plan_impl, const void *host, size_t bytes);
+void topePlanDownload(struct topeFFT *f, void *plan_impl, void *host, size_t bytes);
+
+/* Wait for all work enqueued on f->stream. */
+void topeFFTSynchronize(struct topeFFT *f);
+
+/* Number of kernel launches one Exec performs, and per-launch timing:
+ * while enabled, every Exec records HIP events around each launch on
+ * f->stream; topePlanPassTimes synchronises and returns the mean duration
+ * in milliseconds of each launch position over the recorded execs. */
+int topePlanPassCount(void *plan_impl);
+void topePlanPassTiming(void *plan_impl, int enable);
+int topePlanPassTimes(struct topeFFT *f, void *plan_impl, double *ms, int max_passes);
+/* Kernel name of launch position `pass` (static string). */
+const char *topePlanPassName(void *plan_impl, int pass);
+
+/* Size of the public structs, so foreign-language bindings can check their
+ * mirrors of this header. which: 0 topeFFT, 1 topePlan1D, 2 topePlan2D,
+ * 3 topePlan3D. */
+size_t topeStructSize(int which);
+
+/* Human-readable message for a status code. */
+const char *topeErrorString(int code);
+
+/* Copy the stage twiddle table the plan generated for axis `axis` into host
+ * memory as interleaved doubles (forward sign), returning the entry count;
+ * with host == NULL only the count is returned.  Lets tests pin the GPU's
+ * FP64 twiddle generation against exact values. */
+long long topePlanTwiddles(struct topeFFT *f, void *plan_impl, int axis, double *host, long long max_entries);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TOPEFFT_H */

@@ -1,0 +1,84 @@
+"""The C-ABI boundary, checked without a GPU: the library loads, exports
+every function include/topefft.h declares, the ctypes mirrors match the
+header's struct sizes, the header compiles as C, the drop-in harness
+(reference call sequence of test/2d.c) compiles and links, and
+topeFFTInit reports a missing device instead of aborting."""
+import os
+import re
+import subprocess
+
+import pytest
+
+import topefft
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(REPO, "include", "topefft.h")
+LIBDIR = os.path.join(REPO, "tope-fft_amd", "lib")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"^[A-Za-z_][\w \*]*?\b(\w+)\s*\([^;{]*\)\s*;", src, flags=re.M)
+    return sorted(set(names))
+
+
+def test_header_declares_reference_entry_points():
+    names = declared_functions()
+    for ref in ["topeFFTInit", "tope1DPlanInit", "tope1DExec", "tope1DDestroy", "tope2DPlanInit",
+                "tope2DExec", "tope2DDestroy", "tope3DPlanInit", "tope3DExec", "tope3DDestroy",
+                "profileThis"]:
+        assert ref in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = topefft.load()
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    # and the binding describes every one of them
+    assert set(declared_functions()) <= set(topefft.SIGNATURES)
+
+
+def test_struct_layout_matches_header():
+    mine, theirs = topefft.check_layout()
+    assert mine == theirs
+
+
+def test_constants_match_reference():
+    src = open(HEADER).read()
+    # reference src/topefft.h:8-11
+    for name, val in (("C2C", 1), ("R2C", 0), ("FORWARD", 1), ("INVERSE", 0)):
+        assert re.search(rf"#define {name} {val}\b", src)
+    assert "$CHECKERROR" in src
+
+
+def test_header_compiles_as_c(tmp_path):
+    c = tmp_path / "t.c"
+    c.write_text('#include "topefft.h"\nint main(void){struct topeFFT fw; struct topeFFT *f=&fw;'
+                 ' f->error=TOPE_SUCCESS; $CHECKERROR struct topePlan2D p; (void)p; return 0;}\n')
+    subprocess.run(["gcc", "-std=gnu99", "-Wall", "-Werror", "-I", os.path.dirname(HEADER), str(c), "-o",
+                    str(tmp_path / "t")], check=True)
+
+
+def test_dropin_harness_builds_and_links(tmp_path):
+    exe = tmp_path / "harness2d"
+    subprocess.run(["gcc", "-std=gnu99", "-O2", "-Wall", "-I", os.path.dirname(HEADER),
+                    os.path.join(REPO, "tope-fft_amd", "examples", "harness2d.c"), "-o", str(exe),
+                    "-L", LIBDIR, "-ltopefft", "-lm", f"-Wl,-rpath,{LIBDIR}"], check=True)
+    assert exe.exists()
+
+
+def test_init_without_device_reports_error():
+    # This container has no GPU: topeFFTInit must set f->error, not abort
+    # (the reference asserts at src/util.c:28-31).
+    if os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK | os.W_OK):
+        pytest.skip("a GPU is visible")
+    with pytest.raises(topefft.TopeError) as e:
+        topefft.Framework()
+    assert e.value.code == topefft.TOPE_ERROR_NO_DEVICE
+
+
+def test_error_strings():
+    lib = topefft.load()
+    assert b"size" in lib.topeErrorString(topefft.TOPE_ERROR_INVALID_SIZE)
+    assert lib.topeErrorString(0) == b"success"

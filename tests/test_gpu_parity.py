@@ -1,0 +1,248 @@
+"""Parity of the HIP path (through the C-ABI) with the CPU oracle.
+
+Bar (FP64 path): relative L2 <= 1e-12 against the oracle on the same seeded
+inputs, for forward and inverse, at every size the oracle finishes in
+seconds; at the benchmark sizes (2D 4096^2, 3D 256^3) against the direct DFT
+at sampled and planted bins, plus size-independent properties (round trip,
+Parseval).  Calls follow the reference's sequence: topeFFTInit,
+tope*PlanInit (uploads), tope*Exec (transforms on the device, downloads),
+tope*Destroy.
+"""
+import numpy as np
+import pytest
+
+import topefft
+import workloads
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def run(fw, dims, x, direction):
+    p = topefft.Plan(fw, dims, x)
+    try:
+        return p.exec(direction)
+    finally:
+        p.destroy()
+
+
+@pytest.fixture
+def gfw():
+    f = topefft.Framework()
+    yield f
+    f.destroy()
+
+
+def seeded(dims, salt=0):
+    n = int(np.prod(dims))
+    h = 0x7E57 + salt
+    for d in dims:
+        h = (h * 1000003 + d) & 0xFFFFFFFF
+    return workloads.complex_uniform(h, n)
+
+
+POW2_1D = [1 << l for l in range(0, 13)]  # 1 .. 4096: single fused pass
+
+
+@pytest.mark.parametrize("n", POW2_1D)
+def test_1d_pow2_fused(gfw, oracle, n):
+    x = seeded((n,))
+    for d in (topefft.FORWARD, topefft.INVERSE):
+        got = run(gfw, (n,), x, d)
+        assert rel_l2(got, oracle.dft(x, (n,), d)) < TOL, (n, d)
+
+
+@pytest.mark.parametrize("n", [1 << 13, 1 << 14, 1 << 16])
+def test_1d_pow2_four_step(gfw, oracle, n):
+    x = seeded((n,))
+    for d in (topefft.FORWARD, topefft.INVERSE):
+        got = run(gfw, (n,), x, d)
+        assert rel_l2(got, oracle.dft(x, (n,), d)) < TOL, (n, d)
+
+
+def test_1d_large_four_step_bins(gfw, oracle):
+    n = 1 << 22
+    x = seeded((n,))
+    rng = np.random.default_rng(5)
+    bins = np.concatenate([[0, 1, n - 1, n // 2], rng.integers(0, n, 24)])
+    fwd = run(gfw, (n,), x, topefft.FORWARD)
+    assert rel_l2(fwd[bins], oracle.dft_bins(x, (n,), bins, 1)) < TOL
+    back = run(gfw, (n,), fwd, topefft.INVERSE)
+    assert rel_l2(back, x) < TOL
+
+
+@pytest.mark.parametrize("n", [3, 5, 6, 7, 12, 17, 100, 243, 1000])
+def test_1d_non_pow2_dft(gfw, oracle, n):
+    x = seeded((n,))
+    for d in (topefft.FORWARD, topefft.INVERSE):
+        got = run(gfw, (n,), x, d)
+        assert rel_l2(got, oracle.dft(x, (n,), d)) < TOL, (n, d)
+
+
+def test_1d_golden_fixtures(gfw, golden):
+    import test_oracle as T
+    for n in [2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 3, 5, 7, 12, 17, 100]:
+        x = workloads.complex_uniform(T.case_seed((n,)), n)
+        assert rel_l2(run(gfw, (n,), x, 1), golden[f"fwd_{n}"]) < TOL
+        assert rel_l2(run(gfw, (n,), x, 0), golden[f"inv_{n}"]) < TOL
+
+
+ND_CASES = [(2, 2), (4, 4), (8, 4), (4, 16), (16, 16), (64, 32), (1, 8), (8, 1), (256, 256), (1024, 16),
+            (16, 1024), (4096, 4), (4, 4096), (2, 2, 2), (4, 4, 4), (8, 16, 4), (16, 8, 32), (32, 32, 32),
+            (1, 4, 8), (64, 4, 8), (4, 64, 8), (8, 4, 64)]
+
+
+@pytest.mark.parametrize("dims", ND_CASES, ids=lambda d: "x".join(map(str, d)))
+def test_nd_against_naive(gfw, oracle, dims):
+    x = seeded(dims)
+    for d in (topefft.FORWARD, topefft.INVERSE):
+        got = run(gfw, dims, x, d)
+        assert rel_l2(got, oracle.dft(x, dims, d)) < TOL, (dims, d)
+
+
+@pytest.mark.parametrize("dims", [(8192, 4), (4, 8192), (16384, 2), (2, 8, 8192), (4, 8192, 2),
+                                  (8192, 2, 2)], ids=lambda d: "x".join(map(str, d)))
+def test_nd_long_axes_four_step(gfw, oracle, dims):
+    x = seeded(dims)
+    for d in (topefft.FORWARD, topefft.INVERSE):
+        got = run(gfw, dims, x, d)
+        assert rel_l2(got, oracle.dft(x, dims, d)) < TOL, (dims, d)
+
+
+def test_nd_golden_fixtures(gfw, golden):
+    import test_oracle as T
+    for dims in T.CASES_ND:
+        x = workloads.complex_uniform(T.case_seed(dims), int(np.prod(dims)))
+        k = T.key(dims)
+        assert rel_l2(run(gfw, dims, x, 1), golden[f"fwd_{k}"]) < TOL
+        assert rel_l2(run(gfw, dims, x, 0), golden[f"inv_{k}"]) < TOL
+
+
+def test_reference_harness_inputs(gfw):
+    import test_oracle as T
+    # ramp of test/1d.c:70 and flat sinusoid of test/2d.c:77 (closed forms)
+    for n in (16, 4096, 1 << 16):
+        x = np.arange(1, n + 1).astype(complex)
+        assert rel_l2(run(gfw, (n,), x, 1), T.ramp_spectrum(n)) < TOL
+    for nx, ny in ((64, 32), (1024, 512)):
+        x = np.sin(2 * np.pi * np.arange(nx * ny) / (nx * ny)).astype(complex)
+        assert rel_l2(run(gfw, (nx, ny), x, 1), T.sinusoid_spectrum(nx, ny)) < 1e-11
+
+
+def test_forward_then_inverse_on_device(gfw):
+    # the reference's Exec does not re-read d: FORWARD then INVERSE round
+    # trips on the device-resident copy (test/2d.c:101-102)
+    dims = (512, 256)
+    x = seeded(dims)
+    p = topefft.Plan(gfw, dims, x)
+    p.exec(topefft.FORWARD)
+    back = p.exec(topefft.INVERSE)
+    p.destroy()
+    assert rel_l2(back, x) < TOL
+
+
+def full_size_check(fw, oracle, dims, seed):
+    dims3 = tuple(list(dims) + [1] * (3 - len(dims)))
+    x, peaks = workloads.planted_input(seed, dims3)
+    n = x.size
+    p = topefft.Plan(fw, dims, x)
+    X = p.exec(topefft.FORWARD)
+    # sampled bins + planted peaks against the direct DFT
+    rng = np.random.default_rng(seed)
+    pb = workloads.peak_bins(dims3, peaks)
+    bins = np.concatenate([pb, rng.integers(0, n, 56), [0, n - 1]])
+    want = oracle.dft_bins(x, dims3, bins, 1)
+    assert rel_l2(X[bins], want) < TOL
+    # each planted exponential shows up at its bin with magnitude ~ n
+    for b, (kx, ky, kz, ph) in zip(pb, peaks):
+        assert abs(X[b] - n * np.exp(1j * ph)) < 0.01 * n
+    # Parseval
+    ex, eX = np.vdot(x, x).real, np.vdot(X, X).real
+    assert abs(eX - n * ex) / (n * ex) < TOL
+    # inverse on the device-resident spectrum returns the input
+    back = p.exec(topefft.INVERSE)
+    p.destroy()
+    assert rel_l2(back, x) < TOL
+    return X
+
+
+def test_full_2d_4096(gfw, oracle):
+    full_size_check(gfw, oracle, (4096, 4096), workloads.SEED_2D)
+
+
+def test_full_3d_256(gfw, oracle):
+    full_size_check(gfw, oracle, (256, 256, 256), workloads.SEED_3D)
+
+
+def test_full_1d_65536_seeded(gfw, oracle):
+    x = workloads.complex_uniform(workloads.SEED_1D, 1 << 16)
+    X = run(gfw, (1 << 16,), x, 1)
+    assert rel_l2(X, oracle.dft(x, (1 << 16,), 1)) < TOL
+    assert rel_l2(run(gfw, (1 << 16,), X, 0), x) < TOL
+
+
+def test_twiddle_table_exact(gfw):
+    # GPU FP64 twiddle generation vs extended-precision values: entry
+    # off[s] + (m-1)*Ns + k = W_{Ns*r}^{k*m}
+    import mpmath
+    mpmath.mp.dps = 30
+    n = 4096
+    p = topefft.Plan(gfw, (n,), seeded((n,)))
+    tw = p.twiddles(0)
+    p.destroy()
+    # schedule of the n=4096 kernel: radix 16,16,16 (E = 16)
+    want = []
+    ns = 16
+    for s in (1, 2):
+        for m in range(1, 16):
+            for k in range(ns):
+                e, M = (k * m) % (ns * 16), ns * 16
+                a = 2 * mpmath.pi * e / M
+                want.append(complex(float(mpmath.cos(a)), -float(mpmath.sin(a))))
+        ns *= 16
+    want = np.array(want)
+    assert tw.shape == want.shape
+    assert np.max(np.abs(tw - want)) < 2.3e-16
+
+
+def test_profile_counters_and_passes(gfw):
+    dims = (4096, 4096)
+    x = seeded((64, 64))  # small host buffer replicated below
+    big = np.tile(x, 4096)
+    p = topefft.Plan(gfw, dims, big)
+    assert p.t.totalMemory > 0
+    assert p.t.totalPreKernel > 0
+    p.exec(topefft.FORWARD)
+    assert p.t.totalKernel > 0
+    assert gfw.lib.profileThis(gfw.f.event) > 0
+    assert p.pass_count() == 2
+    p.pass_timing(True)
+    for _ in range(3):
+        p.exec_device(topefft.FORWARD)
+    ms = p.pass_times()
+    assert len(ms) == 2 and all(m > 0 for m in ms)
+    p.destroy()
+
+
+def test_error_paths(gfw):
+    x = np.zeros(12, complex)
+    with pytest.raises(topefft.TopeError) as e:
+        topefft.Plan(gfw, (12, 1), x)  # 2D/3D need powers of two
+    assert e.value.code == topefft.TOPE_ERROR_INVALID_SIZE
+    p = topefft.Plan(gfw, (16,), np.zeros(16, complex))
+    with pytest.raises(topefft.TopeError) as e:
+        p.exec(5)
+    assert e.value.code == topefft.TOPE_ERROR_INVALID_ARG
+    p.destroy()
+
+
+def test_destroy_last_plan_releases_framework():
+    f = topefft.Framework()
+    p = topefft.Plan(f, (8,), np.ones(8, complex))
+    p.destroy()
+    assert not f.f.impl  # reference src/fft1d.c:385-392 releases the context
+    with pytest.raises(topefft.TopeError) as e:
+        topefft.Plan(f, (8,), np.ones(8, complex))
+    assert e.value.code == topefft.TOPE_ERROR_NOT_INITIALIZED

@@ -1,0 +1,949 @@
+// C-ABI of the topeFFT library for MI355X: framework bootstrap, plan/exec/
+// destroy for 1D/2D/3D complex FP64 transforms, and the device-resident and
+// timing extensions declared in include/topefft.h.
+//
+// Reference counterparts: topeFFTInit src/util.c:21-236, tope1D* src/fft1d.c,
+// tope2D* src/fft2d.c, tope3D* src/fft3d.c.  The reference plans one
+// single-radix launch per stage plus a bit-reversal swap and a divide pass
+// per axis, with a host clFinish after each; a plan here is a short list of
+// fused axis passes (one per axis for lengths up to 4096, two for longer
+// axes via a four-step split) enqueued back to back on one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/topefft.h"
+#include "internal.hpp"
+
+using namespace tope;
+
+namespace {
+
+constexpr int kMaxFused = 4096;          // longest axis one pass holds on chip
+constexpr long long kMaxAxis = 1LL << 24;  // four-step limit (4096 x 4096)
+
+struct EventPair {
+    hipEvent_t start = nullptr, stop = nullptr;
+    bool recorded = false;
+};
+
+struct FrameworkImpl {
+    hipStream_t stream = nullptr;
+    int live_plans = 0;
+    EventPair last;  // last exec's device time (profileThis)
+};
+
+struct Pass {
+    const AxisVariant *var = nullptr;  // nullptr: naive DFT pass
+    AxisArgs<double> a{};
+    unsigned grid = 0;
+    int in_buf = 0, out_buf = 0;
+    long long reps = 1, rep_in = 0, rep_out = 0;  // repeated launches (rare geometry)
+    bool last = false;                            // receives the inverse 1/N scale
+    long long dft_n = 0;
+    const void *dft_roots = nullptr;
+    const char *name = "";
+};
+
+struct PlanImpl {
+    FrameworkImpl *fw = nullptr;
+    int ndim = 0;
+    long long dims[3] = {1, 1, 1};
+    long long total = 0;
+    size_t bytes = 0;
+    void *buf[2] = {nullptr, nullptr};
+    int final_buf = 0;
+    std::vector<Pass> passes;
+    std::vector<void *> allocs;
+    std::map<std::pair<int, int>, void *> stage_tw;  // (n, e) -> table
+    struct AxisTw {
+        int n = 0, e = 0;
+        void *table = nullptr;
+    } axis_tw[3];
+    EventPair run;  // this plan's exec events
+    // per-pass timing
+    bool timing = false;
+    std::vector<hipEvent_t> pool;
+    size_t pool_used = 0;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+};
+
+thread_local std::string g_msg;
+
+int map_err(hipError_t e) { return e == hipSuccess ? TOPE_SUCCESS : (int)e; }
+
+FrameworkImpl *fw_of(struct topeFFT *f)
+{
+    if (!f || !f->impl) return nullptr;
+    return static_cast<FrameworkImpl *>(f->impl);
+}
+
+bool is_pow2(long long n) { return n >= 1 && (n & (n - 1)) == 0; }
+int ilog2ll(long long n)
+{
+    int l = 0;
+    while ((1LL << l) < n) ++l;
+    return l;
+}
+
+// Reference radix rule and stage count (src/fft1d.c:316-329, :366-369;
+// src/fft2d.c:335-354, :435-439), kept in the public plan fields.
+void ref_radix(long long n, int *radix, int *stages)
+{
+    if (!is_pow2(n)) {
+        *radix = -1;
+        *stages = 0;
+        return;
+    }
+    int l = ilog2ll(n);
+    if (l % 3 == 0) {
+        *radix = 8;
+        *stages = l / 3;
+    } else if (l % 2 == 0) {
+        *radix = 4;
+        *stages = l / 2;
+    } else {
+        *radix = 2;
+        *stages = l;
+    }
+}
+
+hipError_t ensure_event(hipEvent_t *e)
+{
+    if (*e) return hipSuccess;
+    return hipEventCreate(e);
+}
+
+const AxisVariant *find_variant_named(const char *name)
+{
+    int cnt = 0;
+    const AxisVariant *v = axis_variants(&cnt);
+    for (int i = 0; i < cnt; ++i)
+        if (std::strcmp(v[i].name, name) == 0) return &v[i];
+    return nullptr;
+}
+
+// Variant choice: rows use the single row variant for n; strided tiles take
+// the widest tile that the line count fills (wider tiles give longer
+// contiguous segments per row), preferring the full-complex exchange at equal
+// width.  TOPEFFT_VARIANT_<n>_<row|col>=<name> overrides (used by tuning).
+const AxisVariant *select_variant(int prec, int n, bool cfast, long long lines)
+{
+    char key[64];
+    std::snprintf(key, sizeof key, "TOPEFFT_VARIANT_%d_%s", n, cfast ? "col" : "row");
+    if (const char *ov = std::getenv(key)) {
+        const AxisVariant *v = find_variant_named(ov);
+        if (v && v->prec == prec && v->n == n) return v;
+    }
+    int cnt = 0;
+    const AxisVariant *vs = axis_variants(&cnt);
+    const AxisVariant *best = nullptr;
+    for (int i = 0; i < cnt; ++i) {
+        const AxisVariant *v = &vs[i];
+        if (v->prec != prec || v->n != n || v->cfast != cfast) continue;
+        if (!best) {
+            best = v;
+            continue;
+        }
+        const bool fits = v->tb <= lines, bfits = best->tb <= lines;
+        if (fits && !bfits) best = v;
+        else if (fits == bfits) {
+            if (fits) {
+                if (v->tb > best->tb || (v->tb == best->tb && !v->split && best->split)) best = v;
+            } else if (v->tb < best->tb) {
+                best = v;
+            }
+        }
+    }
+    return best;
+}
+
+struct Geo {  // lines (o < O, c < C) of length n, element stride is
+    long long O, os, C, cs, is;
+};
+
+// Line enumeration for a four-step step: the axis' own lines (o, c) plus one
+// new index (count, input stride, output stride).  Two of the three go into
+// the kernel's (O, C) slots, C taking the smallest output stride so strided
+// tiles stay contiguous; a third non-trivial index becomes repeated launches.
+struct Fold {
+    Geo in{1, 0, 1, 0, 0}, out{1, 0, 1, 0, 0};
+    long long reps = 1, rep_in = 0, rep_out = 0;
+    bool new_in_o = false, new_in_c = false;
+};
+
+Fold fold(const Geo &g, long long cnt, long long s_in, long long s_out)
+{
+    struct Idx {
+        long long count, si, so;
+        bool fresh;
+    };
+    std::vector<Idx> v;
+    if (g.O > 1) v.push_back({g.O, g.os, g.os, false});
+    if (g.C > 1) v.push_back({g.C, g.cs, g.cs, false});
+    if (cnt > 1) v.push_back({cnt, s_in, s_out, true});
+    Fold r;
+    if (v.size() == 3) {  // the axis' outer index becomes repeated launches
+        r.reps = v[0].count;
+        r.rep_in = v[0].si;
+        r.rep_out = v[0].so;
+        v.erase(v.begin());
+    }
+    if (v.size() == 2 && v[0].so < v[1].so) std::swap(v[0], v[1]);
+    if (v.size() == 2) {
+        r.in.O = r.out.O = v[0].count;
+        r.in.os = v[0].si;
+        r.out.os = v[0].so;
+        r.new_in_o = v[0].fresh;
+        v.erase(v.begin());
+    }
+    if (v.size() == 1) {
+        r.in.C = r.out.C = v[0].count;
+        r.in.cs = v[0].si;
+        r.out.cs = v[0].so;
+        r.new_in_c = v[0].fresh;
+    }
+    return r;
+}
+
+class Planner {
+   public:
+    Planner(PlanImpl *p, hipStream_t s) : P(p), st(s) {}
+
+    hipError_t stage_table(int n, int e, void **out)
+    {
+        auto key = std::make_pair(n, e);
+        auto it = P->stage_tw.find(key);
+        if (it != P->stage_tw.end()) {
+            *out = it->second;
+            return hipSuccess;
+        }
+        HostSched h = host_sched(n, e);
+        void *d = nullptr;
+        size_t sz = (size_t)(h.twsize > 0 ? h.twsize : 1) * 16;
+        hipError_t err = hipMalloc(&d, sz);
+        if (err != hipSuccess) return err;
+        P->allocs.push_back(d);
+        err = gen_stage_twiddles(d, PREC_F64, n, e, st);
+        if (err != hipSuccess) return err;
+        P->stage_tw[key] = d;
+        *out = d;
+        return hipSuccess;
+    }
+
+    hipError_t power_table(long long M, long long mult, long long count, void **out)
+    {
+        void *d = nullptr;
+        hipError_t err = hipMalloc(&d, (size_t)count * 16);
+        if (err != hipSuccess) return err;
+        P->allocs.push_back(d);
+        err = gen_power_table(d, PREC_F64, M, mult, count, st);
+        *out = d;
+        return err;
+    }
+
+    // One fused pass: lines (O, C) of length n; input geometry gi in buffer
+    // bi, output geometry go in buffer bo.
+    int add_pass(int n, const Geo &gi, const Geo &go, int bi, int bo, long long reps, long long rep_in,
+                 long long rep_out, const void *pt_lo = nullptr, const void *pt_hi = nullptr, long long pt_o = 0,
+                 long long pt_c = 0, long long pt_n = 0, int pt_shift = 0, int axis = -1)
+    {
+        const bool cfast = !(gi.is == 1 && go.is == 1);
+        const AxisVariant *v = select_variant(PREC_F64, n, cfast, gi.C);
+        if (!v) return TOPE_ERROR_NO_KERNEL;
+        HostSched h = host_sched(v->n, v->e);
+        if (h.twsize != v->twsize) return TOPE_ERROR_NO_KERNEL;  // host/device layouts disagree
+        void *tw = nullptr;
+        hipError_t err = stage_table(v->n, v->e, &tw);
+        if (err != hipSuccess) return map_err(err);
+        if (axis >= 0 && !P->axis_tw[axis].table) {
+            P->axis_tw[axis].n = v->n;
+            P->axis_tw[axis].e = v->e;
+            P->axis_tw[axis].table = tw;
+        }
+        Pass ps;
+        ps.var = v;
+        ps.name = v->name;
+        AxisArgs<double> &a = ps.a;
+        a.in_os = gi.os;
+        a.in_cs = gi.cs;
+        a.in_is = gi.is;
+        a.out_os = go.os;
+        a.out_cs = go.cs;
+        a.out_is = go.is;
+        a.C = (int)gi.C;
+        a.tiles = (int)((gi.C + v->tb - 1) / v->tb);
+        a.tw = static_cast<const cpx<double> *>(tw);
+        a.pt_lo = static_cast<const cpx<double> *>(pt_lo);
+        a.pt_hi = static_cast<const cpx<double> *>(pt_hi);
+        a.pt_o = pt_o;
+        a.pt_c = pt_c;
+        a.pt_n = pt_n;
+        a.pt_shift = pt_shift;
+        a.scale = 1.0;
+        const long long grid = gi.O * (long long)a.tiles;
+        if (grid <= 0 || grid > 0x7fffffffLL) return TOPE_ERROR_INVALID_SIZE;
+        ps.grid = (unsigned)grid;
+        ps.in_buf = bi;
+        ps.out_buf = bo;
+        ps.reps = reps;
+        ps.rep_in = rep_in;
+        ps.rep_out = rep_out;
+        P->passes.push_back(ps);
+        return TOPE_SUCCESS;
+    }
+
+    // Transform along an axis of length n whose lines are g; the data is in
+    // buffer `cur` and ends in buffer `cur` (updated for out-of-place steps).
+    int axis(long long n, const Geo &g, int *cur, int axis_id)
+    {
+        if (n == 1) return TOPE_SUCCESS;
+        if (n <= kMaxFused) return add_pass((int)n, g, g, *cur, *cur, 1, 0, 0, nullptr, nullptr, 0, 0, 0, 0, axis_id);
+        if (n > kMaxAxis) return TOPE_ERROR_INVALID_SIZE;
+        // four-step: i = n1 + N1*n2.  Step 1 transforms n2 (length N2, stride
+        // N1*is) per (line, n1), multiplying output k2 by W_n^{n1*k2}; step 2
+        // transforms n1 (length N1, stride is) per (line, k2) and writes
+        // X[k2 + N2*k1] out of place.
+        const int L = ilog2ll(n);
+        const long long N1 = 1LL << ((L + 1) / 2), N2 = n / N1;
+        const int lo_bits = (L + 1) / 2;
+        void *lo = nullptr, *hi = nullptr;
+        hipError_t err = power_table(n, 1, 1LL << lo_bits, &lo);
+        if (err != hipSuccess) return map_err(err);
+        err = power_table(n, 1LL << lo_bits, n >> lo_bits, &hi);
+        if (err != hipSuccess) return map_err(err);
+
+        // step 1 (in place): lines are (old o, old c, n1)
+        Fold f1 = fold(g, N1, g.is, g.is);
+        Geo s1i = f1.in, s1o = f1.out;
+        s1i.is = s1o.is = N1 * g.is;
+        int rc = add_pass((int)N2, s1i, s1o, *cur, *cur, f1.reps, f1.rep_in, f1.rep_out, lo, hi,
+                          f1.new_in_o ? 1 : 0, f1.new_in_c ? 1 : 0, n, lo_bits, axis_id);
+        if (rc) return rc;
+        // step 2 (out of place): lines are (old o, old c, k2)
+        Fold f2 = fold(g, N2, N1 * g.is, g.is);
+        Geo s2i = f2.in, s2o = f2.out;
+        s2i.is = g.is;
+        s2o.is = N2 * g.is;
+        rc = add_pass((int)N1, s2i, s2o, *cur, *cur ^ 1, f2.reps, f2.rep_in, f2.rep_out);
+        if (rc) return rc;
+        *cur ^= 1;
+        return TOPE_SUCCESS;
+    }
+
+   private:
+    PlanImpl *P;
+    hipStream_t st;
+};
+
+hipError_t time_start(EventPair &ev, hipStream_t s)
+{
+    hipError_t e = ensure_event(&ev.start);
+    if (e == hipSuccess) e = ensure_event(&ev.stop);
+    if (e == hipSuccess) e = hipEventRecord(ev.start, s);
+    return e;
+}
+
+hipError_t time_stop(EventPair &ev, hipStream_t s, tope_ulong *acc_ns)
+{
+    hipError_t e = hipEventRecord(ev.stop, s);
+    if (e != hipSuccess) return e;
+    ev.recorded = true;
+    if (acc_ns) {
+        e = hipEventSynchronize(ev.stop);
+        if (e != hipSuccess) return e;
+        float ms = 0;
+        e = hipEventElapsedTime(&ms, ev.start, ev.stop);
+        if (e != hipSuccess) return e;
+        *acc_ns += (tope_ulong)((double)ms * 1e6);
+    }
+    return hipSuccess;
+}
+
+hipEvent_t pool_event(PlanImpl *P)
+{
+    if (P->pool_used == P->pool.size()) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        P->pool.push_back(e);
+    }
+    return P->pool[P->pool_used++];
+}
+
+// Enqueue all passes of one transform on stream s.
+int enqueue(PlanImpl *P, hipStream_t s, int dir)
+{
+    if (dir != FORWARD && dir != INVERSE) return TOPE_ERROR_INVALID_ARG;
+    const int sign = (dir == FORWARD) ? -1 : 1;
+    const double sc = (dir == INVERSE) ? 1.0 / (double)P->total : 1.0;
+    for (size_t i = 0; i < P->passes.size(); ++i) {
+        const Pass &ps = P->passes[i];
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (P->timing) {
+            e0 = pool_event(P);
+            e1 = pool_event(P);
+            if (!e0 || !e1) return TOPE_ERROR_UNSUPPORTED;
+            (void)hipEventRecord(e0, s);
+        }
+        const char *ib = static_cast<const char *>(P->buf[ps.in_buf]);
+        char *ob = static_cast<char *>(P->buf[ps.out_buf]);
+        if (ps.var) {
+            AxisArgs<double> a = ps.a;
+            a.scale = ps.last ? sc : 1.0;
+            for (long long r = 0; r < ps.reps; ++r)
+                ps.var->fn(ib + r * ps.rep_in * 16, ob + r * ps.rep_out * 16, &a, ps.grid, s, sign);
+        } else {
+            hipError_t err = launch_naive_dft(PREC_F64, ib, ob, ps.dft_roots, ps.dft_n, sign, ps.last ? sc : 1.0, s);
+            if (err != hipSuccess) return map_err(err);
+        }
+        hipError_t err = hipGetLastError();
+        if (err != hipSuccess) return map_err(err);
+        if (P->timing) {
+            (void)hipEventRecord(e1, s);
+            P->pending.push_back({(int)i, {e0, e1}});
+        }
+    }
+    if (P->final_buf == 1) std::swap(P->buf[0], P->buf[1]);
+    return TOPE_SUCCESS;
+}
+
+template <typename Plan>
+void sync_fields(Plan *t, PlanImpl *P)
+{
+    t->data = P->buf[0];
+    t->scratch = P->buf[1];
+}
+
+PlanImpl *impl_of(void *p) { return static_cast<PlanImpl *>(p); }
+
+// Common plan construction.  Returns a status; on success *out owns the
+// device buffers and twiddles and the data has been uploaded.
+int build_plan(struct topeFFT *f, int ndim, long long nx, long long ny, long long nz, const double *d,
+               PlanImpl **out, tope_ulong *pre_ns, tope_ulong *mem_ns)
+{
+    *out = nullptr;
+    FrameworkImpl *fw = fw_of(f);
+    if (!fw) return TOPE_ERROR_NOT_INITIALIZED;
+    if (nx < 1 || ny < 1 || nz < 1) return TOPE_ERROR_INVALID_SIZE;
+    const long long total = nx * ny * nz;
+    if (total > (1LL << 34)) return TOPE_ERROR_INVALID_SIZE;
+    if (ndim > 1 && (!is_pow2(nx) || !is_pow2(ny) || !is_pow2(nz))) return TOPE_ERROR_INVALID_SIZE;
+    if (!d) return TOPE_ERROR_INVALID_ARG;
+
+    PlanImpl *P = new (std::nothrow) PlanImpl();
+    if (!P) return TOPE_ERROR_UNSUPPORTED;
+    P->fw = fw;
+    P->ndim = ndim;
+    P->dims[0] = nx;
+    P->dims[1] = ny;
+    P->dims[2] = nz;
+    P->total = total;
+    P->bytes = (size_t)total * 16;
+    hipStream_t s = fw->stream;
+
+    auto fail = [&](int rc) {
+        for (void *a : P->allocs) (void)hipFree(a);
+        if (P->buf[0]) (void)hipFree(P->buf[0]);
+        if (P->buf[1]) (void)hipFree(P->buf[1]);
+        delete P;
+        return rc;
+    };
+
+    hipError_t err = hipMalloc(&P->buf[0], P->bytes);
+    if (err != hipSuccess) return fail(map_err(err));
+
+    EventPair pre;
+    err = time_start(pre, s);
+    if (err != hipSuccess) return fail(map_err(err));
+
+    Planner pl(P, s);
+    int cur = 0, rc = TOPE_SUCCESS;
+    if (ndim == 1 && !is_pow2(nx)) {
+        // non-power-of-two 1D: GPU DFT with exactly reduced phases
+        void *roots = nullptr;
+        err = hipMalloc(&roots, (size_t)nx * 16);
+        if (err != hipSuccess) return fail(map_err(err));
+        P->allocs.push_back(roots);
+        err = gen_power_table(roots, PREC_F64, nx, 1, nx, s);
+        if (err != hipSuccess) return fail(map_err(err));
+        Pass ps;
+        ps.dft_n = nx;
+        ps.dft_roots = roots;
+        ps.in_buf = 0;
+        ps.out_buf = 1;
+        ps.name = "naive_dft_f64";
+        P->passes.push_back(ps);
+        cur = 1;
+        P->axis_tw[0].n = (int)nx;
+        P->axis_tw[0].e = 0;
+        P->axis_tw[0].table = roots;
+    } else {
+        // X: rows; Y: columns of each plane; Z: columns of the whole volume
+        rc = pl.axis(nx, Geo{1, 0, ny * nz, nx, 1}, &cur, 0);
+        if (!rc && ndim >= 2) rc = pl.axis(ny, Geo{nz, nx * ny, nx, 1, nx}, &cur, 1);
+        if (!rc && ndim >= 3) rc = pl.axis(nz, Geo{1, 0, nx * ny, 1, nx * ny}, &cur, 2);
+    }
+    if (rc) return fail(rc);
+    if (!P->passes.empty()) P->passes.back().last = true;
+    P->final_buf = cur;
+    bool need_scratch = false;
+    for (const Pass &ps : P->passes)
+        if (ps.in_buf == 1 || ps.out_buf == 1) need_scratch = true;
+    if (need_scratch) {
+        err = hipMalloc(&P->buf[1], P->bytes);
+        if (err != hipSuccess) return fail(map_err(err));
+    }
+    err = time_stop(pre, s, pre_ns);
+    if (err != hipSuccess) return fail(map_err(err));
+
+    EventPair up;
+    err = time_start(up, s);
+    if (err == hipSuccess) err = hipMemcpyAsync(P->buf[0], d, P->bytes, hipMemcpyHostToDevice, s);
+    if (err == hipSuccess) err = time_stop(up, s, mem_ns);
+    (void)hipEventDestroy(pre.start);
+    (void)hipEventDestroy(pre.stop);
+    (void)hipEventDestroy(up.start);
+    (void)hipEventDestroy(up.stop);
+    if (err != hipSuccess) return fail(map_err(err));
+    *out = P;
+    return TOPE_SUCCESS;
+}
+
+void free_plan(PlanImpl *P)
+{
+    if (!P) return;
+    if (P->fw && P->fw->stream) (void)hipStreamSynchronize(P->fw->stream);
+    for (void *a : P->allocs) (void)hipFree(a);
+    if (P->buf[0]) (void)hipFree(P->buf[0]);
+    if (P->buf[1]) (void)hipFree(P->buf[1]);
+    for (hipEvent_t e : P->pool) (void)hipEventDestroy(e);
+    if (P->run.start) (void)hipEventDestroy(P->run.start);
+    if (P->run.stop) (void)hipEventDestroy(P->run.stop);
+    delete P;
+}
+
+void release_framework(struct topeFFT *f)
+{
+    FrameworkImpl *fw = fw_of(f);
+    if (!fw) return;
+    if (fw->stream) {
+        (void)hipStreamSynchronize(fw->stream);
+        (void)hipStreamDestroy(fw->stream);
+    }
+    if (fw->last.start) (void)hipEventDestroy(fw->last.start);
+    if (fw->last.stop) (void)hipEventDestroy(fw->last.stop);
+    delete fw;
+    f->impl = nullptr;
+    f->stream = nullptr;
+    f->event = nullptr;
+}
+
+// Exec with the reference's semantics: transform, then blocking copy of the
+// result into d (src/fft2d.c:193-198).
+int exec_host(struct topeFFT *f, PlanImpl *P, double *d, int dir, tope_ulong *kern_ns, tope_ulong *mem_ns)
+{
+    FrameworkImpl *fw = fw_of(f);
+    if (!fw || P->fw != fw) return TOPE_ERROR_NOT_INITIALIZED;
+    if (!d) return TOPE_ERROR_INVALID_ARG;
+    hipStream_t s = fw->stream;
+    hipError_t err = time_start(fw->last, s);
+    if (err != hipSuccess) return map_err(err);
+    int rc = enqueue(P, s, dir);
+    if (rc) return rc;
+    err = time_stop(fw->last, s, kern_ns);
+    if (err != hipSuccess) return map_err(err);
+    EventPair dn;
+    err = time_start(dn, s);
+    if (err == hipSuccess) err = hipMemcpyAsync(d, P->buf[0], P->bytes, hipMemcpyDeviceToHost, s);
+    if (err == hipSuccess) err = time_stop(dn, s, mem_ns);
+    (void)hipEventDestroy(dn.start);
+    (void)hipEventDestroy(dn.stop);
+    return map_err(err);
+}
+
+int exec_device(struct topeFFT *f, PlanImpl *P, int dir)
+{
+    FrameworkImpl *fw = fw_of(f);
+    if (!fw || !P || P->fw != fw) return TOPE_ERROR_NOT_INITIALIZED;
+    return enqueue(P, fw->stream, dir);
+}
+
+void destroy_common(struct topeFFT *f, PlanImpl *P)
+{
+    FrameworkImpl *fw = P ? P->fw : nullptr;
+    free_plan(P);
+    if (fw && fw_of(f) == fw) {
+        if (--fw->live_plans <= 0) release_framework(f);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void topeFFTInit(struct topeFFT *f)
+{
+    if (!f) return;
+    std::memset(f, 0, sizeof(*f));
+    int n = 0;
+    hipError_t err = hipGetDeviceCount(&n);
+    if (err != hipSuccess || n == 0) {
+        f->error = TOPE_ERROR_NO_DEVICE;
+        return;
+    }
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    FrameworkImpl *fw = new (std::nothrow) FrameworkImpl();
+    if (!fw) {
+        f->error = TOPE_ERROR_UNSUPPORTED;
+        return;
+    }
+    err = hipStreamCreateWithFlags(&fw->stream, hipStreamNonBlocking);
+    if (err != hipSuccess) {
+        delete fw;
+        f->error = map_err(err);
+        return;
+    }
+    f->impl = fw;
+    f->stream = fw->stream;
+    f->event = &fw->last;
+    f->device = dev;
+    f->ret_num_platforms = 1;
+    f->ret_num_devices = (unsigned)n;
+    if (std::getenv("TOPEFFT_LOG")) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            std::fprintf(stderr, "topeFFT device %d: %s (%s), %.0f MiB, %d CUs, %zu KiB LDS/block\n", dev, prop.name,
+                         prop.gcnArchName, prop.totalGlobalMem / 1048576.0, prop.multiProcessorCount,
+                         prop.sharedMemPerBlock / 1024);
+    }
+    f->error = TOPE_SUCCESS;
+}
+
+void topeFFTDestroy(struct topeFFT *f)
+{
+    if (!f) return;
+    release_framework(f);
+    f->error = TOPE_SUCCESS;
+}
+
+tope_ulong profileThis(void *event)
+{
+    EventPair *ev = static_cast<EventPair *>(event);
+    if (!ev || !ev->recorded) return 0;
+    if (hipEventSynchronize(ev->stop) != hipSuccess) return 0;
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, ev->start, ev->stop) != hipSuccess) return 0;
+    return (tope_ulong)((double)ms * 1e6);
+}
+
+/* ---- 1D ---------------------------------------------------------------- */
+
+void tope1DPlanInit(struct topeFFT *f, struct topePlan1D *t, int x, int type, double *d)
+{
+    if (!f || !t) return;
+    std::memset(t, 0, sizeof(*t));
+    t->x = x;
+    t->type = type;
+    t->dim = 1;
+    ref_radix(x, &t->radix, &t->log);
+    PlanImpl *P = nullptr;
+    int rc = build_plan(f, 1, x, 1, 1, d, &P, &t->totalPreKernel, &t->totalMemory);
+    f->error = rc;
+    if (rc) return;
+    P->fw->live_plans++;
+    t->impl = P;
+    t->dataSize = P->bytes;
+    t->twiddle = P->axis_tw[0].table;
+    sync_fields(t, P);
+}
+
+void tope1DExec(struct topeFFT *f, struct topePlan1D *t, double *d, int dir)
+{
+    if (!f || !t || !t->impl) {
+        if (f) f->error = TOPE_ERROR_INVALID_ARG;
+        return;
+    }
+    PlanImpl *P = impl_of(t->impl);
+    f->error = exec_host(f, P, d, dir, &t->totalKernel, &t->totalMemory);
+    sync_fields(t, P);
+}
+
+void tope1DExecDevice(struct topeFFT *f, struct topePlan1D *t, int dir)
+{
+    if (!f || !t || !t->impl) {
+        if (f) f->error = TOPE_ERROR_INVALID_ARG;
+        return;
+    }
+    PlanImpl *P = impl_of(t->impl);
+    f->error = exec_device(f, P, dir);
+    sync_fields(t, P);
+}
+
+void tope1DDestroy(struct topeFFT *f, struct topePlan1D *t)
+{
+    if (!f || !t || !t->impl) return;
+    destroy_common(f, impl_of(t->impl));
+    t->impl = nullptr;
+    t->data = t->scratch = t->twiddle = nullptr;
+    f->error = TOPE_SUCCESS;
+}
+
+/* ---- 2D ---------------------------------------------------------------- */
+
+void tope2DPlanInit(struct topeFFT *f, struct topePlan2D *t, int x, int y, int type, double *d)
+{
+    if (!f || !t) return;
+    std::memset(t, 0, sizeof(*t));
+    t->x = x;
+    t->y = y;
+    t->type = type;
+    t->dim = 2;
+    ref_radix(x, &t->radX, &t->logX);
+    ref_radix(y, &t->radY, &t->logY);
+    PlanImpl *P = nullptr;
+    int rc = build_plan(f, 2, x, y, 1, d, &P, &t->totalPreKernel, &t->totalMemory);
+    f->error = rc;
+    if (rc) return;
+    P->fw->live_plans++;
+    t->impl = P;
+    t->dataSize = P->bytes;
+    t->twdX = P->axis_tw[0].table;
+    t->twdY = P->axis_tw[1].table;
+    sync_fields(t, P);
+}
+
+void tope2DExec(struct topeFFT *f, struct topePlan2D *t, double *d, int dir)
+{
+    if (!f || !t || !t->impl) {
+        if (f) f->error = TOPE_ERROR_INVALID_ARG;
+        return;
+    }
+    PlanImpl *P = impl_of(t->impl);
+    f->error = exec_host(f, P, d, dir, &t->totalKernel, &t->totalMemory);
+    sync_fields(t, P);
+}
+
+void tope2DExecDevice(struct topeFFT *f, struct topePlan2D *t, int dir)
+{
+    if (!f || !t || !t->impl) {
+        if (f) f->error = TOPE_ERROR_INVALID_ARG;
+        return;
+    }
+    PlanImpl *P = impl_of(t->impl);
+    f->error = exec_device(f, P, dir);
+    sync_fields(t, P);
+}
+
+void tope2DDestroy(struct topeFFT *f, struct topePlan2D *t)
+{
+    if (!f || !t || !t->impl) return;
+    destroy_common(f, impl_of(t->impl));
+    t->impl = nullptr;
+    t->data = t->scratch = t->twdX = t->twdY = nullptr;
+    f->error = TOPE_SUCCESS;
+}
+
+/* ---- 3D ---------------------------------------------------------------- */
+
+void tope3DPlanInit(struct topeFFT *f, struct topePlan3D *t, int x, int y, int z, int type, double *d)
+{
+    if (!f || !t) return;
+    std::memset(t, 0, sizeof(*t));
+    t->x = x;
+    t->y = y;
+    t->z = z;
+    t->type = type;
+    t->dim = 3;
+    ref_radix(x, &t->radX, &t->logX);
+    ref_radix(y, &t->radY, &t->logY);
+    ref_radix(z, &t->radZ, &t->logZ);
+    PlanImpl *P = nullptr;
+    int rc = build_plan(f, 3, x, y, z, d, &P, &t->totalPreKernel, &t->totalMemory);
+    f->error = rc;
+    if (rc) return;
+    P->fw->live_plans++;
+    t->impl = P;
+    t->dataSize = P->bytes;
+    t->twdX = P->axis_tw[0].table;
+    t->twdY = P->axis_tw[1].table;
+    t->twdZ = P->axis_tw[2].table;
+    sync_fields(t, P);
+}
+
+void tope3DExec(struct topeFFT *f, struct topePlan3D *t, double *d, int dir)
+{
+    if (!f || !t || !t->impl) {
+        if (f) f->error = TOPE_ERROR_INVALID_ARG;
+        return;
+    }
+    PlanImpl *P = impl_of(t->impl);
+    f->error = exec_host(f, P, d, dir, &t->totalKernel, &t->totalMemory);
+    sync_fields(t, P);
+}
+
+void tope3DExecDevice(struct topeFFT *f, struct topePlan3D *t, int dir)
+{
+    if (!f || !t || !t->impl) {
+        if (f) f->error = TOPE_ERROR_INVALID_ARG;
+        return;
+    }
+    PlanImpl *P = impl_of(t->impl);
+    f->error = exec_device(f, P, dir);
+    sync_fields(t, P);
+}
+
+void tope3DDestroy(struct topeFFT *f, struct topePlan3D *t)
+{
+    if (!f || !t || !t->impl) return;
+    destroy_common(f, impl_of(t->impl));
+    t->impl = nullptr;
+    t->data = t->scratch = t->twdX = t->twdY = t->twdZ = nullptr;
+    f->error = TOPE_SUCCESS;
+}
+
+/* ---- extensions -------------------------------------------------------- */
+
+void topePlanUpload(struct topeFFT *f, void *plan_impl, const void *host, size_t bytes)
+{
+    PlanImpl *P = impl_of(plan_impl);
+    FrameworkImpl *fw = fw_of(f);
+    if (!P || !fw || !host || bytes > P->bytes) {
+        if (f) f->error = TOPE_ERROR_INVALID_ARG;
+        return;
+    }
+    hipError_t err = hipMemcpyAsync(P->buf[0], host, bytes, hipMemcpyHostToDevice, fw->stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(fw->stream);
+    f->error = map_err(err);
+}
+
+void topePlanDownload(struct topeFFT *f, void *plan_impl, void *host, size_t bytes)
+{
+    PlanImpl *P = impl_of(plan_impl);
+    FrameworkImpl *fw = fw_of(f);
+    if (!P || !fw || !host || bytes > P->bytes) {
+        if (f) f->error = TOPE_ERROR_INVALID_ARG;
+        return;
+    }
+    hipError_t err = hipMemcpyAsync(host, P->buf[0], bytes, hipMemcpyDeviceToHost, fw->stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(fw->stream);
+    f->error = map_err(err);
+}
+
+void topeFFTSynchronize(struct topeFFT *f)
+{
+    FrameworkImpl *fw = fw_of(f);
+    if (!fw) {
+        if (f) f->error = TOPE_ERROR_NOT_INITIALIZED;
+        return;
+    }
+    f->error = map_err(hipStreamSynchronize(fw->stream));
+}
+
+int topePlanPassCount(void *plan_impl)
+{
+    PlanImpl *P = impl_of(plan_impl);
+    return P ? (int)P->passes.size() : 0;
+}
+
+void topePlanPassTiming(void *plan_impl, int enable)
+{
+    PlanImpl *P = impl_of(plan_impl);
+    if (!P) return;
+    P->timing = enable != 0;
+    if (!P->timing) {
+        P->pending.clear();
+        P->pool_used = 0;
+    }
+}
+
+int topePlanPassTimes(struct topeFFT *f, void *plan_impl, double *ms, int max_passes)
+{
+    PlanImpl *P = impl_of(plan_impl);
+    FrameworkImpl *fw = fw_of(f);
+    if (!P || !fw || !ms) {
+        if (f) f->error = TOPE_ERROR_INVALID_ARG;
+        return 0;
+    }
+    hipError_t err = hipStreamSynchronize(fw->stream);
+    if (err != hipSuccess) {
+        f->error = map_err(err);
+        return 0;
+    }
+    const int np = (int)P->passes.size();
+    std::vector<double> sum(np, 0.0);
+    std::vector<long long> cnt(np, 0);
+    for (auto &pe : P->pending) {
+        float t = 0;
+        if (hipEventElapsedTime(&t, pe.second.first, pe.second.second) == hipSuccess) {
+            sum[pe.first] += t;
+            cnt[pe.first]++;
+        }
+    }
+    P->pending.clear();
+    P->pool_used = 0;
+    const int n = np < max_passes ? np : max_passes;
+    for (int i = 0; i < n; ++i) ms[i] = cnt[i] ? sum[i] / (double)cnt[i] : 0.0;
+    f->error = TOPE_SUCCESS;
+    return n;
+}
+
+const char *topePlanPassName(void *plan_impl, int pass)
+{
+    PlanImpl *P = impl_of(plan_impl);
+    if (!P || pass < 0 || pass >= (int)P->passes.size()) return "";
+    return P->passes[pass].name;
+}
+
+size_t topeStructSize(int which)
+{
+    switch (which) {
+    case 0: return sizeof(struct topeFFT);
+    case 1: return sizeof(struct topePlan1D);
+    case 2: return sizeof(struct topePlan2D);
+    case 3: return sizeof(struct topePlan3D);
+    default: return 0;
+    }
+}
+
+const char *topeErrorString(int code)
+{
+    switch (code) {
+    case TOPE_SUCCESS: return "success";
+    case TOPE_ERROR_INVALID_SIZE: return "unsupported transform size";
+    case TOPE_ERROR_INVALID_ARG: return "invalid argument";
+    case TOPE_ERROR_NO_DEVICE: return "no HIP device";
+    case TOPE_ERROR_NOT_INITIALIZED: return "framework not initialised (topeFFTInit)";
+    case TOPE_ERROR_NO_KERNEL: return "no compiled kernel for this shape";
+    case TOPE_ERROR_UNSUPPORTED: return "unsupported request";
+    default: return code > 0 ? hipGetErrorString((hipError_t)code) : "unknown error";
+    }
+}
+
+long long topePlanTwiddles(struct topeFFT *f, void *plan_impl, int axis, double *host, long long max_entries)
+{
+    PlanImpl *P = impl_of(plan_impl);
+    FrameworkImpl *fw = fw_of(f);
+    if (!P || !fw || axis < 0 || axis > 2 || !P->axis_tw[axis].table) {
+        if (f) f->error = TOPE_ERROR_INVALID_ARG;
+        return -1;
+    }
+    const auto &at = P->axis_tw[axis];
+    long long count = at.e ? host_sched(at.n, at.e).twsize : at.n;
+    if (!host) return count;
+    long long n = count < max_entries ? count : max_entries;
+    hipError_t err = hipMemcpyAsync(host, at.table, (size_t)n * 16, hipMemcpyDeviceToHost, fw->stream);
+    if (err == hipSuccess) err = hipStreamSynchronize(fw->stream);
+    f->error = map_err(err);
+    return err == hipSuccess ? n : -1;
+}
+
+}  // extern "C"

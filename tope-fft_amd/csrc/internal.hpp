@@ -1,0 +1,56 @@
+// Internal interfaces shared by the kernel translation units and the C-ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "axis_kernel.hpp"
+
+namespace tope {
+
+enum { PREC_F64 = 0, PREC_F32 = 1 };
+
+// Launches one axis pass: `args` points to AxisArgs<double> or
+// AxisArgs<float> according to the variant's precision.
+typedef void (*AxisLaunchFn)(const void *in, void *out, const void *args, unsigned grid, hipStream_t s,
+                             int sign);
+
+// One compiled instantiation of axis_kernel.
+struct AxisVariant {
+    int prec;     // PREC_F64 / PREC_F32
+    int n;        // transform length
+    int e;        // elements per thread
+    int tb;       // transforms per block
+    bool cfast;   // lanes walk transforms first (strided-axis tiles)
+    bool split;   // real/imag LDS exchange in two rounds
+    int threads;  // block size
+    int twsize;   // stage twiddle entries for (n, e)
+    int lds_bytes;
+    AxisLaunchFn fn;
+    const char *name;
+};
+
+const AxisVariant *axis_variants(int *count);
+
+// Runtime mirror of Sched<N, E> (the device derives the same layout at
+// compile time; api.cpp checks that both agree for every variant).
+struct HostSched {
+    int nst;
+    int radix[16];
+    int span[16];
+    int off[16];
+    int twsize;
+};
+HostSched host_sched(int n, int e);
+
+// FP64 generation of the stage twiddle table of schedule (n, e): entry
+// off[s] + (m-1)*span[s] + k = W_{span[s]*radix[s]}^{k*m}, forward sign,
+// rounded once to the storage precision.
+hipError_t gen_stage_twiddles(void *out, int prec, int n, int e, hipStream_t s);
+// out[i] = W_M^{i*mult} for i < count (FP64 generation, rounded once).
+hipError_t gen_power_table(void *out, int prec, long long M, long long mult, long long count, hipStream_t s);
+
+// O(n^2) DFT of a length-n vector (any n): out[k] = scale * sum_j in[j]
+// W_n^{sign (j*k mod n)}; `roots` holds W_n^i, i < n (forward sign).
+hipError_t launch_naive_dft(int prec, const void *in, void *out, const void *roots, long long n, int sign,
+                            double scale, hipStream_t s);
+
+}  // namespace tope

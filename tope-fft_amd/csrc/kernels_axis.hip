@@ -1,0 +1,69 @@
+// Compiled instantiations of the fused axis kernel and their registry.
+//
+// Block shapes are chosen for gfx950: wave64, 160 KiB LDS per CU.  Row
+// variants (contiguous axis) put P = N/E consecutive lanes on one transform
+// so each load instruction covers 64 consecutive complex elements.  Column
+// variants (strided axis) put TB transforms on consecutive lanes so a load
+// instruction covers 64/TB rows of TB contiguous elements (TB*16 bytes for
+// FP64); the split-exchange column variants trade two extra barriers per
+// stage for twice as many columns per block.
+#include "internal.hpp"
+
+namespace tope {
+
+template <typename R, int N, int E, int TB, bool CFAST, bool SPLIT>
+static void launch_axis(const void *in, void *out, const void *args, unsigned grid, hipStream_t s, int sign)
+{
+    constexpr int NT = TB * (N / E);
+    const AxisArgs<R> &a = *static_cast<const AxisArgs<R> *>(args);
+    const cpx<R> *i = static_cast<const cpx<R> *>(in);
+    cpx<R> *o = static_cast<cpx<R> *>(out);
+    if (sign < 0)
+        hipLaunchKernelGGL((axis_kernel<R, N, E, TB, CFAST, SPLIT, -1>), dim3(grid), dim3(NT), 0, s, i, o, a);
+    else
+        hipLaunchKernelGGL((axis_kernel<R, N, E, TB, CFAST, SPLIT, 1>), dim3(grid), dim3(NT), 0, s, i, o, a);
+}
+
+template <typename R>
+constexpr int prec_of() { return sizeof(R) == 8 ? PREC_F64 : PREC_F32; }
+
+#define TOPE_LAYOUT_true "_col"
+#define TOPE_LAYOUT_false "_row"
+#define TOPE_SPLIT_true "_split"
+#define TOPE_SPLIT_false ""
+#define TOPE_STR2(x) #x
+#define TOPE_STR(x) TOPE_STR2(x)
+#define VAR(R, N, E, TB, CF, SP)                                                                             \
+    AxisVariant{prec_of<R>(), N, E, TB, CF, SP, TB * (N / E), Sched<N, E>::TWSIZE,                          \
+                Exchange<R, N, E, TB, CF, SP>::BYTES, &launch_axis<R, N, E, TB, CF, SP>,                   \
+                "axis_" TOPE_STR(R) "_n" TOPE_STR(N) "_e" TOPE_STR(E) "_tb" TOPE_STR(TB) TOPE_LAYOUT_##CF \
+                    TOPE_SPLIT_##SP}
+
+// Row variants: 256 threads per block.
+#define ROWS(R)                                                                                     \
+    VAR(R, 2, 2, 256, false, false), VAR(R, 4, 4, 256, false, false), VAR(R, 8, 8, 256, false, false), \
+        VAR(R, 16, 16, 256, false, false), VAR(R, 32, 16, 128, false, false),                         \
+        VAR(R, 64, 16, 64, false, false), VAR(R, 128, 16, 32, false, false),                          \
+        VAR(R, 256, 16, 16, false, false), VAR(R, 512, 16, 8, false, false),                          \
+        VAR(R, 1024, 16, 4, false, false), VAR(R, 2048, 16, 2, false, false),                         \
+        VAR(R, 4096, 16, 1, false, false)
+
+// Column variants: TB columns per block.
+#define COLS(R)                                                                                     \
+    VAR(R, 2, 2, 64, true, false), VAR(R, 4, 4, 64, true, false), VAR(R, 8, 8, 64, true, false),    \
+        VAR(R, 16, 16, 64, true, false), VAR(R, 32, 16, 32, true, false),                           \
+        VAR(R, 64, 16, 16, true, false), VAR(R, 128, 16, 16, true, false),                          \
+        VAR(R, 256, 16, 16, true, false), VAR(R, 512, 16, 8, true, false),                          \
+        VAR(R, 1024, 16, 4, true, false), VAR(R, 1024, 16, 8, true, true),                          \
+        VAR(R, 2048, 16, 2, true, false), VAR(R, 2048, 16, 4, true, true),                          \
+        VAR(R, 4096, 16, 1, true, false), VAR(R, 4096, 16, 2, true, true)
+
+static const AxisVariant kVariants[] = {ROWS(double), COLS(double)};
+
+const AxisVariant *axis_variants(int *count)
+{
+    *count = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
+    return kVariants;
+}
+
+}  // namespace tope

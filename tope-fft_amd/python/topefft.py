@@ -1,0 +1,247 @@
+"""ctypes binding of libtopefft.so (the C-ABI declared in include/topefft.h).
+
+The structs below mirror the header field for field; `check_layout()`
+compares their sizes with what the library reports.  The binding raises if
+the compiled library is missing: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(PKG, "lib", "libtopefft.so")
+
+C2C, R2C, FORWARD, INVERSE = 1, 0, 1, 0
+TOPE_SUCCESS = 0
+TOPE_ERROR_INVALID_SIZE = -1001
+TOPE_ERROR_INVALID_ARG = -1002
+TOPE_ERROR_NO_DEVICE = -1003
+TOPE_ERROR_NOT_INITIALIZED = -1004
+TOPE_ERROR_NO_KERNEL = -1005
+TOPE_ERROR_UNSUPPORTED = -1006
+
+
+class TopeFFT(C.Structure):
+    _fields_ = [("error", C.c_int), ("event", C.c_void_p), ("device", C.c_int),
+                ("stream", C.c_void_p), ("impl", C.c_void_p),
+                ("ret_num_platforms", C.c_uint), ("ret_num_devices", C.c_uint)]
+
+
+_COMMON_TAIL = [("dataSize", C.c_size_t), ("totalMemory", C.c_uint64),
+                ("totalPreKernel", C.c_uint64), ("totalKernel", C.c_uint64),
+                ("dim", C.c_uint), ("impl", C.c_void_p)]
+
+
+class TopePlan1D(C.Structure):
+    _fields_ = [("x", C.c_int), ("log", C.c_int), ("type", C.c_int), ("radix", C.c_int),
+                ("data", C.c_void_p), ("scratch", C.c_void_p), ("twiddle", C.c_void_p)] + _COMMON_TAIL
+
+
+class TopePlan2D(C.Structure):
+    _fields_ = [("x", C.c_int), ("y", C.c_int), ("logX", C.c_int), ("logY", C.c_int),
+                ("type", C.c_int), ("radX", C.c_int), ("radY", C.c_int),
+                ("data", C.c_void_p), ("scratch", C.c_void_p),
+                ("twdX", C.c_void_p), ("twdY", C.c_void_p)] + _COMMON_TAIL
+
+
+class TopePlan3D(C.Structure):
+    _fields_ = [("x", C.c_int), ("y", C.c_int), ("z", C.c_int),
+                ("logX", C.c_int), ("logY", C.c_int), ("logZ", C.c_int),
+                ("type", C.c_int), ("radX", C.c_int), ("radY", C.c_int), ("radZ", C.c_int),
+                ("data", C.c_void_p), ("scratch", C.c_void_p),
+                ("twdX", C.c_void_p), ("twdY", C.c_void_p), ("twdZ", C.c_void_p)] + _COMMON_TAIL
+
+
+# name -> (restype, argtypes)
+_F, _P1, _P2, _P3 = C.POINTER(TopeFFT), C.POINTER(TopePlan1D), C.POINTER(TopePlan2D), C.POINTER(TopePlan3D)
+_DP = C.POINTER(C.c_double)
+SIGNATURES = {
+    "topeFFTInit": (None, [_F]),
+    "topeFFTDestroy": (None, [_F]),
+    "tope1DPlanInit": (None, [_F, _P1, C.c_int, C.c_int, _DP]),
+    "tope1DExec": (None, [_F, _P1, _DP, C.c_int]),
+    "tope1DDestroy": (None, [_F, _P1]),
+    "tope2DPlanInit": (None, [_F, _P2, C.c_int, C.c_int, C.c_int, _DP]),
+    "tope2DExec": (None, [_F, _P2, _DP, C.c_int]),
+    "tope2DDestroy": (None, [_F, _P2]),
+    "tope3DPlanInit": (None, [_F, _P3, C.c_int, C.c_int, C.c_int, C.c_int, _DP]),
+    "tope3DExec": (None, [_F, _P3, _DP, C.c_int]),
+    "tope3DDestroy": (None, [_F, _P3]),
+    "profileThis": (C.c_uint64, [C.c_void_p]),
+    "tope1DExecDevice": (None, [_F, _P1, C.c_int]),
+    "tope2DExecDevice": (None, [_F, _P2, C.c_int]),
+    "tope3DExecDevice": (None, [_F, _P3, C.c_int]),
+    "topePlanUpload": (None, [_F, C.c_void_p, C.c_void_p, C.c_size_t]),
+    "topePlanDownload": (None, [_F, C.c_void_p, C.c_void_p, C.c_size_t]),
+    "topeFFTSynchronize": (None, [_F]),
+    "topePlanPassCount": (C.c_int, [C.c_void_p]),
+    "topePlanPassTiming": (None, [C.c_void_p, C.c_int]),
+    "topePlanPassTimes": (C.c_int, [_F, C.c_void_p, _DP, C.c_int]),
+    "topePlanPassName": (C.c_char_p, [C.c_void_p, C.c_int]),
+    "topeStructSize": (C.c_size_t, [C.c_int]),
+    "topeErrorString": (C.c_char_p, [C.c_int]),
+    "topePlanTwiddles": (C.c_longlong, [_F, C.c_void_p, C.c_int, _DP, C.c_longlong]),
+}
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load the compiled library (raises OSError if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise OSError(f"libtopefft.so not built at {path}; run __graft_entry__.build()")
+        lib = C.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check_layout():
+    lib = load()
+    mine = [C.sizeof(TopeFFT), C.sizeof(TopePlan1D), C.sizeof(TopePlan2D), C.sizeof(TopePlan3D)]
+    theirs = [lib.topeStructSize(i) for i in range(4)]
+    return mine, theirs
+
+
+class TopeError(RuntimeError):
+    def __init__(self, code, what):
+        self.code = code
+        super().__init__(f"{what}: {load().topeErrorString(code).decode()} ({code})")
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_DP)
+
+
+class Framework:
+    """topeFFTInit / topeFFTDestroy."""
+
+    def __init__(self):
+        self.lib = load()
+        self.f = TopeFFT()
+        self.lib.topeFFTInit(C.byref(self.f))
+        if self.f.error:
+            raise TopeError(self.f.error, "topeFFTInit")
+
+    def check(self, what):
+        if self.f.error:
+            raise TopeError(self.f.error, what)
+
+    def synchronize(self):
+        self.lib.topeFFTSynchronize(C.byref(self.f))
+        self.check("topeFFTSynchronize")
+
+    def destroy(self):
+        if self.f.impl:
+            self.lib.topeFFTDestroy(C.byref(self.f))
+
+
+class Plan:
+    """A 1D/2D/3D C2C plan built through tope{1,2,3}DPlanInit.
+
+    `x` is a complex128 array (or interleaved float64) of shape (nz, ny, nx)
+    or flat; PlanInit uploads it, exec() transforms and downloads like the
+    reference, exec_device() transforms on the device only."""
+
+    def __init__(self, fw, dims, x, type_=C2C):
+        self.fw, self.lib = fw, fw.lib
+        self.dims = tuple(int(d) for d in dims)
+        self.ndim = len(self.dims)
+        n = int(np.prod(self.dims))
+        host = np.ascontiguousarray(x, dtype=np.complex128).reshape(-1)
+        if host.size != n:
+            raise ValueError("input size does not match dims")
+        self.n = n
+        self.host = host.view(np.float64)
+        if self.ndim == 1:
+            self.t = TopePlan1D()
+            self.lib.tope1DPlanInit(C.byref(fw.f), C.byref(self.t), self.dims[0], type_, _dptr(self.host))
+        elif self.ndim == 2:
+            self.t = TopePlan2D()
+            self.lib.tope2DPlanInit(C.byref(fw.f), C.byref(self.t), self.dims[0], self.dims[1], type_,
+                                    _dptr(self.host))
+        else:
+            self.t = TopePlan3D()
+            self.lib.tope3DPlanInit(C.byref(fw.f), C.byref(self.t), self.dims[0], self.dims[1],
+                                    self.dims[2], type_, _dptr(self.host))
+        fw.check(f"tope{self.ndim}DPlanInit{self.dims}")
+
+    @property
+    def impl(self):
+        return self.t.impl
+
+    def exec(self, direction=FORWARD, out=None):
+        """Reference Exec: transform on the device and download into `out`
+        (a fresh complex128 array if None)."""
+        if out is None:
+            out = np.empty(self.n, dtype=np.complex128)
+        buf = out.view(np.float64)
+        fn = getattr(self.lib, f"tope{self.ndim}DExec")
+        fn(C.byref(self.fw.f), C.byref(self.t), _dptr(buf), direction)
+        self.fw.check(f"tope{self.ndim}DExec")
+        return out
+
+    def exec_device(self, direction=FORWARD):
+        fn = getattr(self.lib, f"tope{self.ndim}DExecDevice")
+        fn(C.byref(self.fw.f), C.byref(self.t), direction)
+        self.fw.check(f"tope{self.ndim}DExecDevice")
+
+    def upload(self, x):
+        h = np.ascontiguousarray(x, dtype=np.complex128).reshape(-1)
+        self.lib.topePlanUpload(C.byref(self.fw.f), self.impl, h.ctypes.data_as(C.c_void_p), h.nbytes)
+        self.fw.check("topePlanUpload")
+
+    def download(self):
+        out = np.empty(self.n, dtype=np.complex128)
+        self.lib.topePlanDownload(C.byref(self.fw.f), self.impl, out.ctypes.data_as(C.c_void_p), out.nbytes)
+        self.fw.check("topePlanDownload")
+        return out
+
+    def pass_count(self):
+        return self.lib.topePlanPassCount(self.impl)
+
+    def pass_names(self):
+        return [self.lib.topePlanPassName(self.impl, i).decode() for i in range(self.pass_count())]
+
+    def pass_timing(self, enable=True):
+        self.lib.topePlanPassTiming(self.impl, 1 if enable else 0)
+
+    def pass_times(self):
+        n = self.pass_count()
+        ms = np.zeros(max(n, 1))
+        self.lib.topePlanPassTimes(C.byref(self.fw.f), self.impl, _dptr(ms), n)
+        self.fw.check("topePlanPassTimes")
+        return ms[:n]
+
+    def twiddles(self, axis):
+        cnt = self.lib.topePlanTwiddles(C.byref(self.fw.f), self.impl, axis, None, 0)
+        if cnt < 0:
+            self.fw.check("topePlanTwiddles")
+        buf = np.empty(2 * max(cnt, 1))
+        got = self.lib.topePlanTwiddles(C.byref(self.fw.f), self.impl, axis, _dptr(buf), cnt)
+        self.fw.check("topePlanTwiddles")
+        return buf[: 2 * got].view(np.complex128)
+
+    def destroy(self):
+        if self.t.impl:
+            fn = getattr(self.lib, f"tope{self.ndim}DDestroy")
+            fn(C.byref(self.fw.f), C.byref(self.t))
+
+
+def transform(x, dims, direction=FORWARD):
+    """One-shot transform with the reference call sequence
+    (Init, PlanInit, Exec, Destroy) -- test/2d.c:82-114."""
+    fw = Framework()
+    p = Plan(fw, dims, x)
+    try:
+        return p.exec(direction)
+    finally:
+        p.destroy()
+        fw.destroy()
