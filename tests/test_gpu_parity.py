@@ -20,12 +20,10 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
-def run(fw, dims, x, direction):
-    p = topefft.Plan(fw, dims, x)
-    try:
-        return p.exec(direction)
-    finally:
-        p.destroy()
+def run(_fw, dims, x, direction):
+    # the reference's full call sequence per transform (test/2d.c:82-114):
+    # destroying the last plan releases the framework, so each call inits
+    return topefft.transform(x, dims, direction)
 
 
 @pytest.fixture
@@ -131,19 +129,20 @@ def test_reference_harness_inputs(gfw):
         assert rel_l2(run(gfw, (nx, ny), x, 1), T.sinusoid_spectrum(nx, ny)) < 1e-11
 
 
-def test_forward_then_inverse_on_device(gfw):
+def test_forward_then_inverse_on_device():
     # the reference's Exec does not re-read d: FORWARD then INVERSE round
     # trips on the device-resident copy (test/2d.c:101-102)
     dims = (512, 256)
     x = seeded(dims)
-    p = topefft.Plan(gfw, dims, x)
+    p = topefft.Plan(topefft.Framework(), dims, x)
     p.exec(topefft.FORWARD)
     back = p.exec(topefft.INVERSE)
     p.destroy()
     assert rel_l2(back, x) < TOL
 
 
-def full_size_check(fw, oracle, dims, seed):
+def full_size_check(_fw, oracle, dims, seed):
+    fw = topefft.Framework()
     dims3 = tuple(list(dims) + [1] * (3 - len(dims)))
     x, peaks = workloads.planted_input(seed, dims3)
     n = x.size
@@ -183,13 +182,13 @@ def test_full_1d_65536_seeded(gfw, oracle):
     assert rel_l2(run(gfw, (1 << 16,), X, 0), x) < TOL
 
 
-def test_twiddle_table_exact(gfw):
+def test_twiddle_table_exact():
     # GPU FP64 twiddle generation vs extended-precision values: entry
     # off[s] + (m-1)*Ns + k = W_{Ns*r}^{k*m}
     import mpmath
     mpmath.mp.dps = 30
     n = 4096
-    p = topefft.Plan(gfw, (n,), seeded((n,)))
+    p = topefft.Plan(topefft.Framework(), (n,), seeded((n,)))
     tw = p.twiddles(0)
     p.destroy()
     # schedule of the n=4096 kernel: radix 16,16,16 (E = 16)
@@ -207,7 +206,8 @@ def test_twiddle_table_exact(gfw):
     assert np.max(np.abs(tw - want)) < 2.3e-16
 
 
-def test_profile_counters_and_passes(gfw):
+def test_profile_counters_and_passes():
+    gfw = topefft.Framework()
     dims = (4096, 4096)
     x = seeded((64, 64))  # small host buffer replicated below
     big = np.tile(x, 4096)
@@ -226,7 +226,8 @@ def test_profile_counters_and_passes(gfw):
     p.destroy()
 
 
-def test_error_paths(gfw):
+def test_error_paths():
+    gfw = topefft.Framework()
     x = np.zeros(12, complex)
     with pytest.raises(topefft.TopeError) as e:
         topefft.Plan(gfw, (12, 1), x)  # 2D/3D need powers of two
@@ -236,6 +237,7 @@ def test_error_paths(gfw):
         p.exec(5)
     assert e.value.code == topefft.TOPE_ERROR_INVALID_ARG
     p.destroy()
+    gfw.destroy()
 
 
 def test_destroy_last_plan_releases_framework():

@@ -42,7 +42,7 @@ struct FrameworkImpl {
 };
 
 struct Pass {
-    const AxisVariant *var = nullptr;  // nullptr: naive DFT pass
+    const AxisVariant *var = nullptr;  // nullptr: naive DFT pass (entry of either variant list)
     AxisArgs<double> a{};
     unsigned grid = 0;
     int in_buf = 0, out_buf = 0;
@@ -289,6 +289,20 @@ class Planner {
         a.pt_n = pt_n;
         a.pt_shift = pt_shift;
         a.scale = 1.0;
+        a.xcd = (v->cfast && !std::getenv("TOPEFFT_NO_XCD_REMAP")) ? 1 : 0;
+        a.in_elems = a.out_elems = P->total;
+        {
+            // largest byte offset inside one block's footprint, in and out
+            const long long span_in = ((long long)(v->tb - 1) * gi.cs + (long long)(n - 1) * gi.is + 1) * 16;
+            const long long span_out = ((long long)(v->tb - 1) * go.cs + (long long)(n - 1) * go.is + 1) * 16;
+            if (span_in >= 0x7fffffffLL || span_out >= 0x7fffffffLL) {
+                int cnt = 0, cntb = 0;
+                const AxisVariant *base = axis_variants(&cnt);
+                const AxisVariant *bigv = axis_variants_big(&cntb);
+                v = &bigv[v - base];
+                ps.var = v;
+            }
+        }
         const long long grid = gi.O * (long long)a.tiles;
         if (grid <= 0 || grid > 0x7fffffffLL) return TOPE_ERROR_INVALID_SIZE;
         ps.grid = (unsigned)grid;

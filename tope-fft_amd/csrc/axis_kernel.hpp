@@ -28,16 +28,18 @@ namespace tope {
 constexpr int ilog2c(long long n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
 
 // Radix schedule for an N-point transform with E elements per thread:
-// one leading stage of radix N / E^k (if N is not a power of E), then radix-E
-// stages.  Shared by the host (twiddle table layout) and the device.
+// stages of radix RX = min(E, 16), preceded by one smaller stage when N is
+// not a power of RX.  With E = 32 a thread runs two radix-16 butterflies per
+// stage.  Shared by the host (twiddle table layout) and the device.
 template <int N, int E>
 struct Sched {
-    static constexpr int LN = ilog2c(N), LE = ilog2c(E);
-    static constexpr int NFULL = LE ? LN / LE : 0;
-    static constexpr int REM = 1 << (LE ? LN % LE : LN);
+    static constexpr int RX = E < 16 ? E : 16;
+    static constexpr int LN = ilog2c(N), LR = ilog2c(RX);
+    static constexpr int NFULL = LR ? LN / LR : 0;
+    static constexpr int REM = 1 << (LR ? LN % LR : LN);
     static constexpr int NST = NFULL + (REM > 1 ? 1 : 0);
     static constexpr int P = N / E;
-    __host__ __device__ static constexpr int radix(int s) { return (REM > 1 && s == 0) ? REM : E; }
+    __host__ __device__ static constexpr int radix(int s) { return (REM > 1 && s == 0) ? REM : RX; }
     __host__ __device__ static constexpr int span(int s)
     {
         int v = 1;
@@ -70,7 +72,20 @@ struct AxisArgs {
     long long pt_n;
     int pt_shift;
     R scale;    // multiplies the output (1/N on the last inverse pass, else 1)
+    int xcd;    // 1: remap blocks so each XCD walks a contiguous range of tiles
+    long long in_elems, out_elems;  // buffer sizes (bounds of the buffer resources)
 };
+
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  Strided tiles that are neighbours in memory share 128-byte lines, so
+// with remapping block b runs logical tile (b % 8) * (G / 8) + b / 8: the
+// blocks an XCD receives back to back are adjacent tiles.
+__device__ __forceinline__ long long xcd_block(int remap)
+{
+    const long long b = blockIdx.x, G = gridDim.x;
+    if (!remap || (G & 7)) return b;
+    return (b & 7) * (G >> 3) + (b >> 3);
+}
 
 // LDS index of element i of block-local transform t.  One pad slot per 16
 // elements breaks the power-of-two strides of the Stockham scatter.
@@ -85,6 +100,15 @@ __device__ __forceinline__ int lds_index(int t, int i)
 template <int N, int TB>
 constexpr int lds_slots() { return TB * (N + N / 16); }
 
+// The pad map i -> i + (i >> 4) is linear over every access group used
+// here: element p + P*e (P a power of two) and the radix-r scatter
+// base + m*Ns (Ns, r powers of two) never carry across a 16-boundary
+// differently from their offset, so padded(base + d) = padded(base) +
+// d + (d >> 4) with d = P*e or m*Ns.  This lets every LDS access be one
+// base register plus a compile-time offset.
+template <int TB, bool CFAST>
+constexpr int lds_off(int d) { return (d + (d >> 4)) * (CFAST ? TB : 1); }
+
 // LDS exchange of the E register elements of every thread: write element
 // e to slot dst(e), then read slots p + P*e back.  SPLIT exchanges the real
 // and imaginary parts in two rounds, halving the LDS footprint (this is what
@@ -93,6 +117,24 @@ template <typename R, int N, int E, int TB, bool CFAST, bool SPLIT>
 struct Exchange {
     static constexpr int BYTES = lds_slots<N, TB>() * (SPLIT ? sizeof(R) : 2 * sizeof(R));
 };
+// Twiddles W^{k*m}, m = 1..r-1, of one butterfly.  Only the power-of-two
+// exponents m = 1, 2, 4, 8 are read from the table; every other one is the
+// product of two earlier entries (m = 2^a + b, b < 2^a), at most three
+// products deep.  This keeps 4 table loads in flight instead of r-1.
+template <typename R, int r>
+__device__ __forceinline__ void stage_twiddles(cpx<R> (&w)[r], const cpx<R> *__restrict__ t, int Ns)
+{
+    w[0] = cpx<R>{1, 0};
+#pragma unroll
+    for (int m = 1; m < r; m <<= 1) w[m] = t[(m - 1) * Ns];
+#pragma unroll
+    for (int m = 3; m < r; ++m) {
+        if ((m & (m - 1)) == 0) continue;
+        const int hi = 1 << (31 - __builtin_clz(m));
+        w[m] = cmul(w[hi], w[m - hi]);
+    }
+}
+
 template <typename R, int N, int E, int SIGN, int S>
 __device__ __forceinline__ void run_stage(cpx<R> (&v)[E], int p, const cpx<R> *__restrict__ tw)
 {
@@ -109,9 +151,10 @@ __device__ __forceinline__ void run_stage(cpx<R> (&v)[E], int p, const cpx<R> *_
         if constexpr (S > 0) {
             const int j = p + q * P;
             const int k = j & (Ns - 1);
-            const cpx<R> *t = tw + SC::twoff(S) + k;
+            cpx<R> w[r];
+            stage_twiddles<R, r>(w, tw + SC::twoff(S) + k, Ns);
 #pragma unroll
-            for (int m = 1; m < r; ++m) x[m] = twmul<SIGN>(x[m], t[(m - 1) * Ns]);
+            for (int m = 1; m < r; ++m) x[m] = twmul<SIGN>(x[m], w[m]);
         }
         dft<r, SIGN>(x);
 #pragma unroll
@@ -149,46 +192,87 @@ __device__ __forceinline__ void run_stages(cpx<R> (&v)[E], int p, int t, const c
 #pragma unroll
             for (int e = 0; e < E; ++e) v[e] = w[e];
         } else {
+            constexpr int Ns = SC::span(S);
+            const int rbase = lds_index<N, TB, CFAST>(t, p);
+            int wbase[Q];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) wbase[q] = lds_index<N, TB, CFAST>(t, stage_dst<N, E, S>(p + q * P, 0));
             if constexpr (!SPLIT) {
                 cpx<R> *lds = (cpx<R> *)lds_raw;
                 if constexpr (S > 0) __syncthreads();
 #pragma unroll
                 for (int q = 0; q < Q; ++q)
 #pragma unroll
-                    for (int m = 0; m < r; ++m)
-                        lds[lds_index<N, TB, CFAST>(t, stage_dst<N, E, S>(p + q * P, m))] = v[q + m * Q];
+                    for (int m = 0; m < r; ++m) lds[wbase[q] + lds_off<TB, CFAST>(m * Ns)] = v[q + m * Q];
                 __syncthreads();
 #pragma unroll
-                for (int e = 0; e < E; ++e) v[e] = lds[lds_index<N, TB, CFAST>(t, p + P * e)];
+                for (int e = 0; e < E; ++e) v[e] = lds[rbase + lds_off<TB, CFAST>(P * e)];
             } else {
                 R *lds = (R *)lds_raw;
                 if constexpr (S > 0) __syncthreads();
 #pragma unroll
                 for (int q = 0; q < Q; ++q)
 #pragma unroll
-                    for (int m = 0; m < r; ++m)
-                        lds[lds_index<N, TB, CFAST>(t, stage_dst<N, E, S>(p + q * P, m))] = v[q + m * Q].x;
+                    for (int m = 0; m < r; ++m) lds[wbase[q] + lds_off<TB, CFAST>(m * Ns)] = v[q + m * Q].x;
                 __syncthreads();
 #pragma unroll
-                for (int e = 0; e < E; ++e) v[e].x = lds[lds_index<N, TB, CFAST>(t, p + P * e)];
+                for (int e = 0; e < E; ++e) v[e].x = lds[rbase + lds_off<TB, CFAST>(P * e)];
                 __syncthreads();
 #pragma unroll
                 for (int q = 0; q < Q; ++q)
 #pragma unroll
-                    for (int m = 0; m < r; ++m)
-                        lds[lds_index<N, TB, CFAST>(t, stage_dst<N, E, S>(p + q * P, m))] = v[q + m * Q].y;
+                    for (int m = 0; m < r; ++m) lds[wbase[q] + lds_off<TB, CFAST>(m * Ns)] = v[q + m * Q].y;
                 __syncthreads();
 #pragma unroll
-                for (int e = 0; e < E; ++e) v[e].y = lds[lds_index<N, TB, CFAST>(t, p + P * e)];
+                for (int e = 0; e < E; ++e) v[e].y = lds[rbase + lds_off<TB, CFAST>(P * e)];
             }
         }
         run_stages<R, N, E, TB, CFAST, SPLIT, SIGN, S + 1>(v, p, t, tw, lds_raw);
     }
 }
 
-// One launch = one axis pass over TB transforms per block.  The four-step
-// post-twiddle (a.pt_n != 0) and the output scale are runtime-uniform.
-template <typename R, int N, int E, int TB, bool CFAST, bool SPLIT, int SIGN>
+// Buffer resource over [base, base + bytes) (gfx9 raw buffer, no swizzle);
+// accesses past num_records read zero and drop stores.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, long long bytes)
+{
+    const long long lim = 0x7fffffffLL;
+    const int nrec = (int)(bytes < lim ? (bytes > 0 ? bytes : 0) : lim);
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, nrec, 0x00020000);
+}
+
+typedef unsigned int tope_u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int tope_u32x2 __attribute__((ext_vector_type(2)));
+
+template <typename R>
+__device__ __forceinline__ cpx<R> buf_load(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    if constexpr (sizeof(R) == 8) {
+        tope_u32x4 u = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+        return __builtin_bit_cast(cpx<R>, u);
+    } else {
+        tope_u32x2 u = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+        return __builtin_bit_cast(cpx<R>, u);
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ void buf_store(cpx<R> v, __amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    if constexpr (sizeof(R) == 8)
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(tope_u32x4, v), r, voff, soff, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(tope_u32x2, v), r, voff, soff, 0);
+}
+
+// One launch = one axis pass over TB transforms per block.  POST applies
+// the four-step post-twiddle; the output scale is runtime-uniform.
+//
+// Global accesses go through buffer resources based at the block's first
+// element: each thread keeps one 32-bit byte offset and the per-element
+// stride e*P*is*16 is a uniform (SGPR) offset, which keeps address
+// registers out of the way of the 4*E data registers.  Blocks whose
+// footprint exceeds 2^31 bytes run the BIG instantiation (64-bit pointers).
+template <typename R, int N, int E, int TB, bool CFAST, bool SPLIT, int SIGN, bool POST, bool BIG>
 __global__ void __launch_bounds__(TB *(N / E))
 axis_kernel(const cpx<R> *in, cpx<R> *out, const AxisArgs<R> a)
 {
@@ -196,43 +280,70 @@ axis_kernel(const cpx<R> *in, cpx<R> *out, const AxisArgs<R> a)
     constexpr int P = SC::P;
     constexpr bool USE_LDS = (P > 1) && (SC::NST > 1);
     constexpr int LDS_BYTES = USE_LDS ? Exchange<R, N, E, TB, CFAST, SPLIT>::BYTES : 16;
+    constexpr int CB = (int)sizeof(cpx<R>);
     __shared__ __align__(16) unsigned char lds[LDS_BYTES];
 
     const int tid = threadIdx.x;
     const int t = CFAST ? (tid % TB) : (tid / P);
     const int p = CFAST ? (tid / TB) : (tid % P);
-    const long long o = blockIdx.x / a.tiles;
-    const int tile = blockIdx.x - (int)(o * a.tiles);
-    const int c = tile * TB + t;
+    const long long blk = xcd_block(a.xcd);
+    const long long o = blk / a.tiles;
+    const int tile = (int)(blk - o * a.tiles);
+    const int c0 = tile * TB;
+    const int c = c0 + t;
     const bool valid = c < a.C;
 
     cpx<R> v[E];
-    if (valid) {
-        const cpx<R> *src = in + o * a.in_os + (long long)c * a.in_cs;
+    if constexpr (!BIG) {
+        const long long ibase = o * a.in_os + (long long)c0 * a.in_cs;
+        const __amdgpu_buffer_rsrc_t rin = make_rsrc(in + ibase, (a.in_elems - ibase) * CB);
+        const int voff = (int)((t * a.in_cs + (long long)p * a.in_is) * CB);
+        const int sstep = (int)((long long)P * a.in_is * CB);
+        if (valid) {
 #pragma unroll
-        for (int e = 0; e < E; ++e) v[e] = src[(long long)(p + P * e) * a.in_is];
+            for (int e = 0; e < E; ++e) v[e] = buf_load<R>(rin, voff, e * sstep);
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e] = cpx<R>{0, 0};
+        }
     } else {
+        if (valid) {
+            const cpx<R> *src = in + o * a.in_os + (long long)c * a.in_cs;
 #pragma unroll
-        for (int e = 0; e < E; ++e) v[e] = cpx<R>{0, 0};
+            for (int e = 0; e < E; ++e) v[e] = src[(long long)(p + P * e) * a.in_is];
+        } else {
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e] = cpx<R>{0, 0};
+        }
     }
 
     run_stages<R, N, E, TB, CFAST, SPLIT, SIGN, 0>(v, p, t, a.tw, lds);
 
-    if (valid) {
-        if (a.pt_n) {
-            const long long base = o * a.pt_o + (long long)c * a.pt_c;
-            const long long mask = (1LL << a.pt_shift) - 1;
+    if (!valid) return;
+    if constexpr (POST) {
+        const long long base = o * a.pt_o + (long long)c * a.pt_c;
+        const long long mask = (1LL << a.pt_shift) - 1;
 #pragma unroll
-            for (int e = 0; e < E; ++e) {
-                const long long k = p + P * e;
-                const long long ex = (base * k) & (a.pt_n - 1);
-                const cpx<R> w = cmul(a.pt_lo[ex & mask], a.pt_hi[ex >> a.pt_shift]);
-                v[e] = twmul<SIGN>(v[e], w);
-            }
+        for (int e = 0; e < E; ++e) {
+            const long long k = p + P * e;
+            const long long ex = (base * k) & (a.pt_n - 1);
+            const cpx<R> w = cmul(a.pt_lo[ex & mask], a.pt_hi[ex >> a.pt_shift]);
+            v[e] = twmul<SIGN>(v[e], w);
         }
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = scale(v[e], a.scale);
+    if constexpr (!BIG) {
+        const long long obase = o * a.out_os + (long long)c0 * a.out_cs;
+        const __amdgpu_buffer_rsrc_t rout = make_rsrc(out + obase, (a.out_elems - obase) * CB);
+        const int voff = (int)((t * a.out_cs + (long long)p * a.out_is) * CB);
+        const int sstep = (int)((long long)P * a.out_is * CB);
+#pragma unroll
+        for (int e = 0; e < E; ++e) buf_store<R>(v[e], rout, voff, e * sstep);
+    } else {
         cpx<R> *dst = out + o * a.out_os + (long long)c * a.out_cs;
 #pragma unroll
-        for (int e = 0; e < E; ++e) dst[(long long)(p + P * e) * a.out_is] = scale(v[e], a.scale);
+        for (int e = 0; e < E; ++e) dst[(long long)(p + P * e) * a.out_is] = v[e];
     }
 }
 

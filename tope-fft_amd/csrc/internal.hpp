@@ -29,6 +29,8 @@ struct AxisVariant {
 };
 
 const AxisVariant *axis_variants(int *count);
+// Same list, same order, compiled with 64-bit addressing.
+const AxisVariant *axis_variants_big(int *count);
 
 // Runtime mirror of Sched<N, E> (the device derives the same layout at
 // compile time; api.cpp checks that both agree for every variant).

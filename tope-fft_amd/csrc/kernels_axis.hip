@@ -11,18 +11,38 @@
 
 namespace tope {
 
-template <typename R, int N, int E, int TB, bool CFAST, bool SPLIT>
-static void launch_axis(const void *in, void *out, const void *args, unsigned grid, hipStream_t s, int sign)
+template <typename R, int N, int E, int TB, bool CFAST, bool SPLIT, bool BIG>
+static void launch_axis_t(const void *in, void *out, const void *args, unsigned grid, hipStream_t s, int sign)
 {
     constexpr int NT = TB * (N / E);
     const AxisArgs<R> &a = *static_cast<const AxisArgs<R> *>(args);
     const cpx<R> *i = static_cast<const cpx<R> *>(in);
     cpx<R> *o = static_cast<cpx<R> *>(out);
+    if constexpr (CFAST) {
+        // four-step first steps are strided tiles: only they carry the post-twiddle
+        if (a.pt_n) {
+            if (sign < 0)
+                hipLaunchKernelGGL((axis_kernel<R, N, E, TB, CFAST, SPLIT, -1, true, BIG>), dim3(grid), dim3(NT), 0,
+                                   s, i, o, a);
+            else
+                hipLaunchKernelGGL((axis_kernel<R, N, E, TB, CFAST, SPLIT, 1, true, BIG>), dim3(grid), dim3(NT), 0,
+                                   s, i, o, a);
+            return;
+        }
+    }
     if (sign < 0)
-        hipLaunchKernelGGL((axis_kernel<R, N, E, TB, CFAST, SPLIT, -1>), dim3(grid), dim3(NT), 0, s, i, o, a);
+        hipLaunchKernelGGL((axis_kernel<R, N, E, TB, CFAST, SPLIT, -1, false, BIG>), dim3(grid), dim3(NT), 0, s, i,
+                           o, a);
     else
-        hipLaunchKernelGGL((axis_kernel<R, N, E, TB, CFAST, SPLIT, 1>), dim3(grid), dim3(NT), 0, s, i, o, a);
+        hipLaunchKernelGGL((axis_kernel<R, N, E, TB, CFAST, SPLIT, 1, false, BIG>), dim3(grid), dim3(NT), 0, s, i,
+                           o, a);
 }
+
+#ifdef TOPE_AXIS_BIG
+#define TOPE_LAUNCH(R, N, E, TB, CF, SP) &launch_axis_t<R, N, E, TB, CF, SP, true>
+#else
+#define TOPE_LAUNCH(R, N, E, TB, CF, SP) &launch_axis_t<R, N, E, TB, CF, SP, false>
+#endif
 
 template <typename R>
 constexpr int prec_of() { return sizeof(R) == 8 ? PREC_F64 : PREC_F32; }
@@ -35,7 +55,7 @@ constexpr int prec_of() { return sizeof(R) == 8 ? PREC_F64 : PREC_F32; }
 #define TOPE_STR(x) TOPE_STR2(x)
 #define VAR(R, N, E, TB, CF, SP)                                                                             \
     AxisVariant{prec_of<R>(), N, E, TB, CF, SP, TB * (N / E), Sched<N, E>::TWSIZE,                          \
-                Exchange<R, N, E, TB, CF, SP>::BYTES, &launch_axis<R, N, E, TB, CF, SP>,                   \
+                Exchange<R, N, E, TB, CF, SP>::BYTES, TOPE_LAUNCH(R, N, E, TB, CF, SP),                    \
                 "axis_" TOPE_STR(R) "_n" TOPE_STR(N) "_e" TOPE_STR(E) "_tb" TOPE_STR(TB) TOPE_LAYOUT_##CF \
                     TOPE_SPLIT_##SP}
 
@@ -46,7 +66,9 @@ constexpr int prec_of() { return sizeof(R) == 8 ? PREC_F64 : PREC_F32; }
         VAR(R, 64, 16, 64, false, false), VAR(R, 128, 16, 32, false, false),                          \
         VAR(R, 256, 16, 16, false, false), VAR(R, 512, 16, 8, false, false),                          \
         VAR(R, 1024, 16, 4, false, false), VAR(R, 2048, 16, 2, false, false),                         \
-        VAR(R, 4096, 16, 1, false, false)
+        VAR(R, 4096, 16, 1, false, false), VAR(R, 4096, 16, 1, false, true), VAR(R, 4096, 16, 2, false, true), \
+        VAR(R, 2048, 16, 2, false, true), VAR(R, 1024, 16, 4, false, true), VAR(R, 512, 16, 8, false, true),   \
+        VAR(R, 256, 16, 16, false, true)
 
 // Column variants: TB columns per block.
 #define COLS(R)                                                                                     \
@@ -56,11 +78,19 @@ constexpr int prec_of() { return sizeof(R) == 8 ? PREC_F64 : PREC_F32; }
         VAR(R, 256, 16, 16, true, false), VAR(R, 512, 16, 8, true, false),                          \
         VAR(R, 1024, 16, 4, true, false), VAR(R, 1024, 16, 8, true, true),                          \
         VAR(R, 2048, 16, 2, true, false), VAR(R, 2048, 16, 4, true, true),                          \
-        VAR(R, 4096, 16, 1, true, false), VAR(R, 4096, 16, 2, true, true)
+        VAR(R, 4096, 16, 1, true, false), VAR(R, 4096, 16, 2, true, true), VAR(R, 4096, 16, 4, true, true), \
+        VAR(R, 4096, 32, 2, true, false), VAR(R, 4096, 32, 4, true, true), VAR(R, 2048, 32, 4, true, false), \
+        VAR(R, 2048, 32, 8, true, true)
 
 static const AxisVariant kVariants[] = {ROWS(double), COLS(double)};
 
+// The same list is compiled twice: normally (32-bit buffer offsets) and with
+// TOPE_AXIS_BIG (64-bit pointers) for blocks spanning more than 2^31 bytes.
+#ifdef TOPE_AXIS_BIG
+const AxisVariant *axis_variants_big(int *count)
+#else
 const AxisVariant *axis_variants(int *count)
+#endif
 {
     *count = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
     return kVariants;

@@ -59,15 +59,16 @@ __global__ void gen_power_kernel(cpx<R> *out, long long M, long long mult, long 
 HostSched host_sched(int n, int e)
 {
     HostSched h{};
-    int ln = 0, le = 0;
+    const int rx = e < 16 ? e : 16;
+    int ln = 0, lr = 0;
     while ((1 << ln) < n) ++ln;
-    while ((1 << le) < e) ++le;
-    const int nfull = le ? ln / le : 0;
-    const int rem = 1 << (le ? ln % le : ln);
+    while ((1 << lr) < rx) ++lr;
+    const int nfull = lr ? ln / lr : 0;
+    const int rem = 1 << (lr ? ln % lr : ln);
     h.nst = nfull + (rem > 1 ? 1 : 0);
     int span = 1, off = 0;
     for (int s = 0; s < h.nst; ++s) {
-        h.radix[s] = (rem > 1 && s == 0) ? rem : e;
+        h.radix[s] = (rem > 1 && s == 0) ? rem : rx;
         h.span[s] = span;
         h.off[s] = off;
         if (s > 0) off += (h.radix[s] - 1) * span;
