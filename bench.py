@@ -52,6 +52,29 @@ def measure(fw, plan, steps, warmup):
     return (t1 - t0) / steps, pass_ms
 
 
+def kernel_symbol(variant_name, sign=-1):
+    """axis_double_n4096_e16_tb4_col_split -> the kernel's template name
+    prefix as rocprofv3 prints it (forward, no post-twiddle)."""
+    parts = variant_name.split("_")
+    prec, n, e, tb = parts[1], parts[2][1:], parts[3][1:], parts[4][2:]
+    col = "true" if parts[5] == "col" else "false"
+    split = "true" if parts[-1] == "split" else "false"
+    return f"tope::axis_kernel<{prec}, {n}, {e}, {tb}, {col}, {split}, {sign}, false"
+
+
+def profiled_traffic(variant_name):
+    """HBM bytes per launch of this kernel from the committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (scaled by a plain copy), or None."""
+    path = os.path.join(REPO, "profiles", "pmc_traffic_2d4096_fp64.md")
+    if not os.path.exists(path):
+        return None
+    want = kernel_symbol(variant_name)
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch ") and want in line:
+            return int(line.split()[-1])
+    return None
+
+
 def cpu_baseline(x, dims, budget_s=12.0):
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle_py
@@ -141,7 +164,7 @@ def main():
                                    "pass_ms": [round(float(m), 4) for m in pass3]}},
         "roofline": {"bound": "hbm", "kernel": names2[dom], "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None,
+                     "traffic": profiled_traffic(names2[dom]),
                      "per_launch_algorithmic_bytes": dom_bytes,
                      "avg_launch_ms": round(float(pass2[dom]), 5)},
         "cpu_baseline": cpu,

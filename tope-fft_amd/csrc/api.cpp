@@ -116,10 +116,13 @@ void ref_radix(long long n, int *radix, int *stages)
     }
 }
 
+// Timing events skip the system-scope fence (cache writeback/invalidate)
+// that a default event performs when it completes: with it, every event
+// between two passes costs ~10 us of idle GPU.
 hipError_t ensure_event(hipEvent_t *e)
 {
     if (*e) return hipSuccess;
-    return hipEventCreate(e);
+    return hipEventCreateWithFlags(e, hipEventDisableSystemFence);
 }
 
 const AxisVariant *find_variant_named(const char *name)
@@ -131,10 +134,19 @@ const AxisVariant *find_variant_named(const char *name)
     return nullptr;
 }
 
-// Variant choice: rows use the single row variant for n; strided tiles take
-// the widest tile that the line count fills (wider tiles give longer
-// contiguous segments per row), preferring the full-complex exchange at equal
-// width.  TOPEFFT_VARIANT_<n>_<row|col>=<name> overrides (used by tuning).
+// Variants measured fastest on MI355X for their (length, layout) at the
+// benchmark shapes (2D 4096^2: rows 84 us vs 89-93 us for the others,
+// 4-column split tiles 114 us vs 140 us for 2-column tiles).
+const char *const kPreferred[] = {
+    "axis_double_n4096_e16_tb1_row_split",
+    "axis_double_n4096_e16_tb4_col_split",
+};
+
+// Variant choice: a measured preference if the line count fills its tile;
+// otherwise strided tiles take the widest tile that the line count fills
+// (wider tiles give longer contiguous segments per row), preferring the
+// full-complex exchange at equal width.  TOPEFFT_VARIANT_<n>_<row|col>=<name>
+// overrides (used by tuning sweeps).
 const AxisVariant *select_variant(int prec, int n, bool cfast, long long lines)
 {
     char key[64];
@@ -142,6 +154,10 @@ const AxisVariant *select_variant(int prec, int n, bool cfast, long long lines)
     if (const char *ov = std::getenv(key)) {
         const AxisVariant *v = find_variant_named(ov);
         if (v && v->prec == prec && v->n == n) return v;
+    }
+    for (const char *name : kPreferred) {
+        const AxisVariant *v = find_variant_named(name);
+        if (v && v->prec == prec && v->n == n && v->cfast == cfast && v->tb <= lines) return v;
     }
     int cnt = 0;
     const AxisVariant *vs = axis_variants(&cnt);
@@ -386,7 +402,7 @@ hipEvent_t pool_event(PlanImpl *P)
 {
     if (P->pool_used == P->pool.size()) {
         hipEvent_t e = nullptr;
-        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        if (ensure_event(&e) != hipSuccess) return nullptr;
         P->pool.push_back(e);
     }
     return P->pool[P->pool_used++];
