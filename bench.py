@@ -36,19 +36,23 @@ def gbs(nbytes, seconds):
 
 
 def measure(fw, plan, steps, warmup):
+    """Timed region: `steps` transforms back to back on the plan's stream,
+    bracketed by stream synchronisation, alternating FORWARD and INVERSE so
+    the resident data stays bounded (the two directions run the same
+    kernels; INVERSE also applies the fused 1/N).  Then each kernel alone:
+    HIP events on the same stream around `steps` launches of that pass
+    (events between passes would add ~8 us idle gaps to every step)."""
     import topefft
-    for _ in range(warmup):
-        plan.exec_device(topefft.FORWARD)
-    fw.synchronize()
-    plan.pass_timing(True)
+    dirs = [topefft.FORWARD, topefft.INVERSE]
+    for i in range(warmup):
+        plan.exec_device(dirs[i % 2])
     fw.synchronize()
     t0 = time.perf_counter()
-    for _ in range(steps):
-        plan.exec_device(topefft.FORWARD)
+    for i in range(steps):
+        plan.exec_device(dirs[i % 2])
     fw.synchronize()
     t1 = time.perf_counter()
-    pass_ms = plan.pass_times()
-    plan.pass_timing(False)
+    pass_ms = np.array([plan.time_pass(i, topefft.FORWARD, steps) for i in range(plan.pass_count())])
     return (t1 - t0) / steps, pass_ms
 
 
@@ -73,6 +77,42 @@ def profiled_traffic(variant_name):
         if line.startswith("traffic_bytes_per_launch ") and want in line:
             return int(line.split()[-1])
     return None
+
+
+def measure_batch(fw, x, n, count, prec, mode, steps, warmup):
+    """One batch plan: first FORWARD downloaded (accuracy), then timed."""
+    import topefft
+    p = topefft.BatchPlan(fw, n, count, x, prec, mode)
+    out = p.exec(topefft.FORWARD)
+    t, pass_ms = measure(fw, p, steps, warmup)
+    name = p.pass_names()[0]
+    p.destroy()
+    return out, t, pass_ms, name
+
+
+def mixed_precision_leg(fw, steps, warmup):
+    """4096 transforms of length 4096: complex128 FP64 against complex64 with
+    radix-16 stages on MFMA (and on the VALU), Gaussian input."""
+    import topefft
+    import workloads
+    n = count = 4096
+    x = workloads.complex_gaussian(workloads.SEED_BATCH, n * count)
+    ref, t64, p64, name64 = measure_batch(fw, x, n, count, topefft.PREC_F64, topefft.MODE_VALU, steps, warmup)
+    refn = np.linalg.norm(ref)
+    res = {"workload": "batch 4096 x 1D C2C n=4096, Gaussian(0,1), forward",
+           "fp64": {"kernel": name64, "ms": round(t64 * 1e3, 4), "transforms_per_s": round(count / t64, 1),
+                    "gbs": round(gbs(2 * 16 * n * count, t64), 1)},
+           "complex64": {}}
+    x32 = x.astype(np.complex64)
+    for mode in (topefft.MODE_MFMA_BF16X6, topefft.MODE_MFMA_FP32, topefft.MODE_MFMA_BF16X3, topefft.MODE_VALU):
+        out, t, pm, name = measure_batch(fw, x32, n, count, topefft.PREC_F32, mode, steps, warmup)
+        d = out.astype(np.complex128) - ref
+        res["complex64"][topefft.MODE_NAMES[mode]] = {
+            "kernel": name, "ms": round(t * 1e3, 4), "transforms_per_s": round(count / t, 1),
+            "gbs": round(gbs(2 * 8 * n * count, t), 1), "speedup_vs_fp64": round(t64 / t, 3),
+            "rel_l2_vs_fp64": float(f"{np.linalg.norm(d) / refn:.3e}"),
+            "max_abs_vs_fp64": float(f"{np.max(np.abs(d)):.3e}")}
+    return res
 
 
 def cpu_baseline(x, dims, budget_s=12.0):
@@ -131,6 +171,8 @@ def main():
     p3.destroy()
     del x3
 
+    mp = mixed_precision_leg(fw, max(5, args.steps // 2), args.warmup)
+
     cpu = None
     if not args.no_cpu:
         ct, reps, cores = cpu_baseline(x2, dims2)
@@ -161,7 +203,8 @@ def main():
                    "passes": {n: round(float(m), 4) for n, m in zip(names2, pass2)},
                    "3d_256cubed": {"gbs": round(gbs(alg3, t3), 2), "ms": round(t3 * 1e3, 4),
                                    "transforms_per_s": round(1.0 / t3, 2),
-                                   "pass_ms": [round(float(m), 4) for m in pass3]}},
+                                   "pass_ms": [round(float(m), 4) for m in pass3]},
+                   "mixed_precision_batch": mp},
         "roofline": {"bound": "hbm", "kernel": names2[dom], "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": profiled_traffic(names2[dom]),

@@ -174,12 +174,55 @@ void topeFFTSynchronize(struct topeFFT *f);
 int topePlanPassCount(void *plan_impl);
 void topePlanPassTiming(void *plan_impl, int enable);
 int topePlanPassTimes(struct topeFFT *f, void *plan_impl, double *ms, int max_passes);
+/* Mean device time in milliseconds of launch position `pass` alone: HIP
+ * events around `reps` back-to-back launches of that pass on f->stream (no
+ * events between the launches).  Only in-place passes; returns -1 and sets
+ * f->error otherwise. */
+double topePlanTimePass(struct topeFFT *f, void *plan_impl, int pass, int dir, int reps);
 /* Kernel name of launch position `pass` (static string). */
 const char *topePlanPassName(void *plan_impl, int pass);
 
+/* ---- batched 1D transforms (no reference counterpart) -------------------
+ * `count` independent length-`n` C2C transforms stored back to back.  FP64
+ * plans hold interleaved complex128 (host pointer: double*), FP32 plans
+ * complex64 (host pointer: float*).  FP32 plans choose how the radix-16
+ * stages run:
+ *   TOPE_MODE_VALU        FP32 butterflies on the vector ALUs
+ *   TOPE_MODE_MFMA_BF16X3 16x16 DFT-matrix products on MFMA, operands split
+ *                         into 2 bf16 parts, 3 products (~1e-5 rel-L2)
+ *   TOPE_MODE_MFMA_BF16X6 3 bf16 parts, 6 products (~FP32 accuracy)
+ *   TOPE_MODE_MFMA_FP32   plain FP32 operands on MFMA
+ * MFMA modes take n = 256 or 4096.  Twiddles and DFT-matrix constants are
+ * generated in FP64 on the GPU and rounded once; INVERSE scales by 1/n. */
+#define TOPE_PREC_F64 0
+#define TOPE_PREC_F32 1
+#define TOPE_MODE_VALU 0
+#define TOPE_MODE_MFMA_BF16X3 1
+#define TOPE_MODE_MFMA_BF16X6 2
+#define TOPE_MODE_MFMA_FP32 3
+
+struct topePlanBatch {
+    int n;
+    int count;
+    int precision;
+    int mode;
+    void *data; /* device buffer, count * n complex elements */
+    size_t dataSize;
+    tope_ulong totalMemory;
+    tope_ulong totalPreKernel;
+    tope_ulong totalKernel;
+    void *impl;
+};
+
+void topeBatchPlanInit(struct topeFFT *f, struct topePlanBatch *t, int n, int count, int precision, int mode,
+                       const void *host);
+void topeBatchExec(struct topeFFT *f, struct topePlanBatch *t, void *host, int dir);
+void topeBatchExecDevice(struct topeFFT *f, struct topePlanBatch *t, int dir);
+void topeBatchDestroy(struct topeFFT *f, struct topePlanBatch *t);
+
 /* Size of the public structs, so foreign-language bindings can check their
  * mirrors of this header. which: 0 topeFFT, 1 topePlan1D, 2 topePlan2D,
- * 3 topePlan3D. */
+ * 3 topePlan3D, 4 topePlanBatch. */
 size_t topeStructSize(int which);
 
 /* Human-readable message for a status code. */

@@ -1,0 +1,100 @@
+"""Batched 1D transforms: FP64 and the FP32-storage variants (VALU and the
+MFMA DFT-matrix modes) against the FP64 oracle.
+
+Bars: FP64 rel-L2 <= 1e-12.  FP32 storage: plain FP32 arithmetic (VALU and
+MFMA fp32 operands) and the 3-part bf16 split <= 1e-6; the 2-part bf16 split
+(dropped terms ~2^-16) <= 1e-4.  The measured errors are printed.
+"""
+import numpy as np
+import pytest
+
+import topefft
+import workloads
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+MODE_TOL = {topefft.MODE_VALU: 1e-6, topefft.MODE_MFMA_FP32: 1e-6, topefft.MODE_MFMA_BF16X6: 1e-6,
+            topefft.MODE_MFMA_BF16X3: 1e-4}
+
+
+def batch_input(n, count, seed):
+    return workloads.complex_gaussian(seed, n * count)
+
+
+def run_batch(n, count, x, prec, mode, direction):
+    fw = topefft.Framework()
+    p = topefft.BatchPlan(fw, n, count, x, prec, mode)
+    try:
+        return p.exec(direction)
+    finally:
+        p.destroy()
+        fw.destroy()
+
+
+@pytest.mark.parametrize("n", [2, 16, 256, 1024, 4096, 8192])
+def test_batch_fp64(oracle, n):
+    count = 12
+    x = batch_input(n, count, 100 + n)
+    for d in (topefft.FORWARD, topefft.INVERSE):
+        got = run_batch(n, count, x, topefft.PREC_F64, topefft.MODE_VALU, d).reshape(count, n)
+        for r in (0, 5, count - 1):
+            assert rel_l2(got[r], oracle.dft(x[r * n:(r + 1) * n], (n,), d)) < 1e-12
+
+
+@pytest.mark.parametrize("mode", [topefft.MODE_VALU, topefft.MODE_MFMA_BF16X3, topefft.MODE_MFMA_BF16X6,
+                                  topefft.MODE_MFMA_FP32], ids=lambda m: topefft.MODE_NAMES[m])
+@pytest.mark.parametrize("n", [256, 4096])
+def test_batch_fp32_modes(oracle, n, mode):
+    count = 40
+    x = batch_input(n, count, 200 + n).astype(np.complex64)
+    x64 = x.astype(np.complex128)
+    for d in (topefft.FORWARD, topefft.INVERSE):
+        got = run_batch(n, count, x, topefft.PREC_F32, mode, d).reshape(count, n).astype(np.complex128)
+        errs = []
+        for r in (0, 1, 17, count - 1):
+            want = oracle.dft(x64[r * n:(r + 1) * n], (n,), d)
+            errs.append(rel_l2(got[r], want))
+        print(f"n={n} mode={topefft.MODE_NAMES[mode]} dir={d} rel-L2 max {max(errs):.3e}")
+        assert max(errs) < MODE_TOL[mode], errs
+
+
+def test_batch_fp32_full_4096(oracle):
+    # the benchmark batch: 4096 transforms of 4096, Gaussian input; every
+    # MFMA mode against the FP64 GPU batch (itself pinned to the oracle)
+    n = count = 4096
+    x = workloads.complex_gaussian(workloads.SEED_BATCH, n * count)
+    ref = run_batch(n, count, x, topefft.PREC_F64, topefft.MODE_VALU, topefft.FORWARD)
+    xs = x.reshape(count, n)
+    for r in (0, 2048, 4095):
+        assert rel_l2(ref.reshape(count, n)[r], oracle.dft(xs[r], (n,), 1)) < 1e-12
+    x32 = x.astype(np.complex64)
+    for mode, tol in MODE_TOL.items():
+        got = run_batch(n, count, x32, topefft.PREC_F32, mode, topefft.FORWARD).astype(np.complex128)
+        err = rel_l2(got, ref)
+        mx = np.max(np.abs(got - ref))
+        print(f"4096x4096 mode={topefft.MODE_NAMES[mode]} rel-L2 vs FP64 {err:.3e} max-abs {mx:.3e}")
+        assert err < tol
+
+
+def test_batch_round_trip_mfma():
+    n, count = 4096, 64
+    x = batch_input(n, count, 7).astype(np.complex64)
+    fw = topefft.Framework()
+    p = topefft.BatchPlan(fw, n, count, x, topefft.PREC_F32, topefft.MODE_MFMA_BF16X6)
+    p.exec(topefft.FORWARD)
+    back = p.exec(topefft.INVERSE)
+    p.destroy()
+    fw.destroy()
+    assert rel_l2(back.astype(np.complex128), x.astype(np.complex128)) < 2e-6
+
+
+def test_batch_rejects_unsupported():
+    fw = topefft.Framework()
+    x = np.zeros(512 * 2, np.complex64)
+    with pytest.raises(topefft.TopeError) as e:
+        topefft.BatchPlan(fw, 512, 2, x, topefft.PREC_F32, topefft.MODE_MFMA_BF16X6)
+    assert e.value.code == topefft.TOPE_ERROR_UNSUPPORTED
+    with pytest.raises(topefft.TopeError):
+        topefft.BatchPlan(fw, 256, 2, np.zeros(512, complex), topefft.PREC_F64, topefft.MODE_MFMA_FP32)
+    fw.destroy()

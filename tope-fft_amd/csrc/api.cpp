@@ -22,6 +22,7 @@
 
 #include "../../include/topefft.h"
 #include "internal.hpp"
+#include "mfma_kernel.hpp"
 
 using namespace tope;
 
@@ -41,9 +42,15 @@ struct FrameworkImpl {
     EventPair last;  // last exec's device time (profileThis)
 };
 
+enum PassKind { PASS_AXIS = 0, PASS_DFT = 1, PASS_MFMA = 2 };
+
 struct Pass {
-    const AxisVariant *var = nullptr;  // nullptr: naive DFT pass (entry of either variant list)
-    AxisArgs<double> a{};
+    int kind = PASS_AXIS;
+    const AxisVariant *var = nullptr;  // PASS_AXIS: entry of either variant list
+    AxisArgs<double> a{};              // FP64 axis pass
+    AxisArgs<float> a32{};             // FP32 axis pass
+    MfmaArgs m{};                      // PASS_MFMA
+    int mfma_mode = 0, mfma_n = 0;
     unsigned grid = 0;
     int in_buf = 0, out_buf = 0;
     long long reps = 1, rep_in = 0, rep_out = 0;  // repeated launches (rare geometry)
@@ -55,6 +62,8 @@ struct Pass {
 
 struct PlanImpl {
     FrameworkImpl *fw = nullptr;
+    int prec = PREC_F64;
+    size_t elem = 16;  // bytes per complex element
     int ndim = 0;
     long long dims[3] = {1, 1, 1};
     long long total = 0;
@@ -63,7 +72,7 @@ struct PlanImpl {
     int final_buf = 0;
     std::vector<Pass> passes;
     std::vector<void *> allocs;
-    std::map<std::pair<int, int>, void *> stage_tw;  // (n, e) -> table
+    std::map<std::pair<int, int>, void *> stage_tw;  // (n, e + 64 * prec) -> table
     struct AxisTw {
         int n = 0, e = 0;
         void *table = nullptr;
@@ -236,7 +245,7 @@ class Planner {
 
     hipError_t stage_table(int n, int e, void **out)
     {
-        auto key = std::make_pair(n, e);
+        auto key = std::make_pair(n, e + 64 * P->prec);
         auto it = P->stage_tw.find(key);
         if (it != P->stage_tw.end()) {
             *out = it->second;
@@ -244,11 +253,11 @@ class Planner {
         }
         HostSched h = host_sched(n, e);
         void *d = nullptr;
-        size_t sz = (size_t)(h.twsize > 0 ? h.twsize : 1) * 16;
+        size_t sz = (size_t)(h.twsize > 0 ? h.twsize : 1) * P->elem;
         hipError_t err = hipMalloc(&d, sz);
         if (err != hipSuccess) return err;
         P->allocs.push_back(d);
-        err = gen_stage_twiddles(d, PREC_F64, n, e, st);
+        err = gen_stage_twiddles(d, P->prec, n, e, st);
         if (err != hipSuccess) return err;
         P->stage_tw[key] = d;
         *out = d;
@@ -258,10 +267,10 @@ class Planner {
     hipError_t power_table(long long M, long long mult, long long count, void **out)
     {
         void *d = nullptr;
-        hipError_t err = hipMalloc(&d, (size_t)count * 16);
+        hipError_t err = hipMalloc(&d, (size_t)count * P->elem);
         if (err != hipSuccess) return err;
         P->allocs.push_back(d);
-        err = gen_power_table(d, PREC_F64, M, mult, count, st);
+        err = gen_power_table(d, P->prec, M, mult, count, st);
         *out = d;
         return err;
     }
@@ -273,11 +282,12 @@ class Planner {
                  long long pt_c = 0, long long pt_n = 0, int pt_shift = 0, int axis = -1)
     {
         const bool cfast = !(gi.is == 1 && go.is == 1);
-        const AxisVariant *v = select_variant(PREC_F64, n, cfast, gi.C);
+        const AxisVariant *v = select_variant(P->prec, n, cfast, gi.C);
         if (!v) return TOPE_ERROR_NO_KERNEL;
         HostSched h = host_sched(v->n, v->e);
         if (h.twsize != v->twsize) return TOPE_ERROR_NO_KERNEL;  // host/device layouts disagree
         void *tw = nullptr;
+        // (kind stays PASS_AXIS)
         hipError_t err = stage_table(v->n, v->e, &tw);
         if (err != hipSuccess) return map_err(err);
         if (axis >= 0 && !P->axis_tw[axis].table) {
@@ -309,8 +319,8 @@ class Planner {
         a.in_elems = a.out_elems = P->total;
         {
             // largest byte offset inside one block's footprint, in and out
-            const long long span_in = ((long long)(v->tb - 1) * gi.cs + (long long)(n - 1) * gi.is + 1) * 16;
-            const long long span_out = ((long long)(v->tb - 1) * go.cs + (long long)(n - 1) * go.is + 1) * 16;
+            const long long span_in = ((long long)(v->tb - 1) * gi.cs + (long long)(n - 1) * gi.is + 1) * (long long)P->elem;
+            const long long span_out = ((long long)(v->tb - 1) * go.cs + (long long)(n - 1) * go.is + 1) * (long long)P->elem;
             if (span_in >= 0x7fffffffLL || span_out >= 0x7fffffffLL) {
                 int cnt = 0, cntb = 0;
                 const AxisVariant *base = axis_variants(&cnt);
@@ -318,6 +328,27 @@ class Planner {
                 v = &bigv[v - base];
                 ps.var = v;
             }
+        }
+        {
+            AxisArgs<float> &b = ps.a32;
+            b.in_os = a.in_os;
+            b.in_cs = a.in_cs;
+            b.in_is = a.in_is;
+            b.out_os = a.out_os;
+            b.out_cs = a.out_cs;
+            b.out_is = a.out_is;
+            b.C = a.C;
+            b.tiles = a.tiles;
+            b.tw = static_cast<const cpx<float> *>(tw);
+            b.pt_lo = static_cast<const cpx<float> *>(pt_lo);
+            b.pt_hi = static_cast<const cpx<float> *>(pt_hi);
+            b.pt_o = pt_o;
+            b.pt_c = pt_c;
+            b.pt_n = pt_n;
+            b.pt_shift = pt_shift;
+            b.scale = 1.0f;
+            b.xcd = a.xcd;
+            b.in_elems = b.out_elems = P->total;
         }
         const long long grid = gi.O * (long long)a.tiles;
         if (grid <= 0 || grid > 0x7fffffffLL) return TOPE_ERROR_INVALID_SIZE;
@@ -408,14 +439,44 @@ hipEvent_t pool_event(PlanImpl *P)
     return P->pool[P->pool_used++];
 }
 
+// Launch pass i of a plan on stream s.
+int launch_pass(PlanImpl *P, size_t i, hipStream_t s, int dir)
+{
+    const int sign = (dir == FORWARD) ? -1 : 1;
+    const double sc = (dir == INVERSE) ? 1.0 / (double)P->total : 1.0;
+    const Pass &ps = P->passes[i];
+    const char *ib = static_cast<const char *>(P->buf[ps.in_buf]);
+    char *ob = static_cast<char *>(P->buf[ps.out_buf]);
+    const long long eb = (long long)P->elem;
+    if (ps.kind == PASS_AXIS) {
+        if (ps.var->prec == PREC_F64) {
+            AxisArgs<double> a = ps.a;
+            a.scale = ps.last ? sc : 1.0;
+            for (long long r = 0; r < ps.reps; ++r)
+                ps.var->fn(ib + r * ps.rep_in * eb, ob + r * ps.rep_out * eb, &a, ps.grid, s, sign);
+        } else {
+            AxisArgs<float> a = ps.a32;
+            a.scale = ps.last ? (float)sc : 1.0f;
+            for (long long r = 0; r < ps.reps; ++r)
+                ps.var->fn(ib + r * ps.rep_in * eb, ob + r * ps.rep_out * eb, &a, ps.grid, s, sign);
+        }
+    } else if (ps.kind == PASS_MFMA) {
+        MfmaArgs m = ps.m;
+        m.scale = ps.last ? (float)sc : 1.0f;  // 1/N is a power of two: exact in FP32
+        hipError_t err = launch_mfma_rows(ps.mfma_mode, ps.mfma_n, ib, ob, m, sign, s);
+        if (err != hipSuccess) return map_err(err);
+    } else {
+        hipError_t err = launch_naive_dft(P->prec, ib, ob, ps.dft_roots, ps.dft_n, sign, ps.last ? sc : 1.0, s);
+        if (err != hipSuccess) return map_err(err);
+    }
+    return map_err(hipGetLastError());
+}
+
 // Enqueue all passes of one transform on stream s.
 int enqueue(PlanImpl *P, hipStream_t s, int dir)
 {
     if (dir != FORWARD && dir != INVERSE) return TOPE_ERROR_INVALID_ARG;
-    const int sign = (dir == FORWARD) ? -1 : 1;
-    const double sc = (dir == INVERSE) ? 1.0 / (double)P->total : 1.0;
     for (size_t i = 0; i < P->passes.size(); ++i) {
-        const Pass &ps = P->passes[i];
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (P->timing) {
             e0 = pool_event(P);
@@ -423,19 +484,8 @@ int enqueue(PlanImpl *P, hipStream_t s, int dir)
             if (!e0 || !e1) return TOPE_ERROR_UNSUPPORTED;
             (void)hipEventRecord(e0, s);
         }
-        const char *ib = static_cast<const char *>(P->buf[ps.in_buf]);
-        char *ob = static_cast<char *>(P->buf[ps.out_buf]);
-        if (ps.var) {
-            AxisArgs<double> a = ps.a;
-            a.scale = ps.last ? sc : 1.0;
-            for (long long r = 0; r < ps.reps; ++r)
-                ps.var->fn(ib + r * ps.rep_in * 16, ob + r * ps.rep_out * 16, &a, ps.grid, s, sign);
-        } else {
-            hipError_t err = launch_naive_dft(PREC_F64, ib, ob, ps.dft_roots, ps.dft_n, sign, ps.last ? sc : 1.0, s);
-            if (err != hipSuccess) return map_err(err);
-        }
-        hipError_t err = hipGetLastError();
-        if (err != hipSuccess) return map_err(err);
+        int rc = launch_pass(P, i, s, dir);
+        if (rc) return rc;
         if (P->timing) {
             (void)hipEventRecord(e1, s);
             P->pending.push_back({(int)i, {e0, e1}});
@@ -454,10 +504,14 @@ void sync_fields(Plan *t, PlanImpl *P)
 
 PlanImpl *impl_of(void *p) { return static_cast<PlanImpl *>(p); }
 
-// Common plan construction.  Returns a status; on success *out owns the
-// device buffers and twiddles and the data has been uploaded.
-int build_plan(struct topeFFT *f, int ndim, long long nx, long long ny, long long nz, const double *d,
-               PlanImpl **out, tope_ulong *pre_ns, tope_ulong *mem_ns)
+enum { SHAPE_BATCH = 4 };
+
+// Common plan construction.  shape 1/2/3: an nx*ny*nz FP64 array (the
+// reference plans); SHAPE_BATCH: ny rows of length nx in precision `prec`
+// run by `mode`.  Returns a status; on success *out owns the device buffers
+// and twiddles and the data has been uploaded.
+int build_plan(struct topeFFT *f, int shape, long long nx, long long ny, long long nz, int prec, int mode,
+               const void *d, PlanImpl **out, tope_ulong *pre_ns, tope_ulong *mem_ns)
 {
     *out = nullptr;
     FrameworkImpl *fw = fw_of(f);
@@ -465,18 +519,25 @@ int build_plan(struct topeFFT *f, int ndim, long long nx, long long ny, long lon
     if (nx < 1 || ny < 1 || nz < 1) return TOPE_ERROR_INVALID_SIZE;
     const long long total = nx * ny * nz;
     if (total > (1LL << 34)) return TOPE_ERROR_INVALID_SIZE;
-    if (ndim > 1 && (!is_pow2(nx) || !is_pow2(ny) || !is_pow2(nz))) return TOPE_ERROR_INVALID_SIZE;
+    if ((shape == 2 || shape == 3) && (!is_pow2(nx) || !is_pow2(ny) || !is_pow2(nz))) return TOPE_ERROR_INVALID_SIZE;
+    if (shape == SHAPE_BATCH && !is_pow2(nx)) return TOPE_ERROR_INVALID_SIZE;
+    if (prec != PREC_F64 && prec != PREC_F32) return TOPE_ERROR_INVALID_ARG;
+    if (mode != MP_VALU && (prec != PREC_F32 || shape != SHAPE_BATCH)) return TOPE_ERROR_UNSUPPORTED;
+    if (mode != MP_VALU && mode != MP_BF16X3 && mode != MP_BF16X6 && mode != MP_FP32) return TOPE_ERROR_INVALID_ARG;
+    if (mode != MP_VALU && !mfma_supported((int)nx)) return TOPE_ERROR_UNSUPPORTED;
     if (!d) return TOPE_ERROR_INVALID_ARG;
 
     PlanImpl *P = new (std::nothrow) PlanImpl();
     if (!P) return TOPE_ERROR_UNSUPPORTED;
     P->fw = fw;
-    P->ndim = ndim;
+    P->prec = prec;
+    P->elem = prec == PREC_F64 ? 16 : 8;
+    P->ndim = shape;
     P->dims[0] = nx;
     P->dims[1] = ny;
     P->dims[2] = nz;
     P->total = total;
-    P->bytes = (size_t)total * 16;
+    P->bytes = (size_t)total * P->elem;
     hipStream_t s = fw->stream;
 
     auto fail = [&](int rc) {
@@ -496,7 +557,32 @@ int build_plan(struct topeFFT *f, int ndim, long long nx, long long ny, long lon
 
     Planner pl(P, s);
     int cur = 0, rc = TOPE_SUCCESS;
-    if (ndim == 1 && !is_pow2(nx)) {
+    if (shape == SHAPE_BATCH && mode != MP_VALU) {
+        // one MFMA launch over all rows, in place
+        void *tw = nullptr, *kc = nullptr;
+        err = pl.stage_table((int)nx, 16, &tw);
+        if (err == hipSuccess) err = hipMalloc(&kc, mfma_constants_bytes());
+        if (err != hipSuccess) return fail(map_err(err));
+        P->allocs.push_back(kc);
+        err = gen_mfma_constants(kc, s);
+        if (err != hipSuccess) return fail(map_err(err));
+        Pass ps;
+        ps.kind = PASS_MFMA;
+        ps.mfma_mode = mode;
+        ps.mfma_n = (int)nx;
+        ps.m.count = ny;
+        ps.m.tw = static_cast<const cpx<float> *>(tw);
+        ps.m.k = static_cast<const MfmaConsts *>(kc);
+        ps.m.scale = 1.0f;
+        ps.m.in_elems = ps.m.out_elems = total;
+        ps.name = mode == MP_BF16X3 ? "mfma_rows_bf16x3" : mode == MP_BF16X6 ? "mfma_rows_bf16x6" : "mfma_rows_fp32";
+        P->passes.push_back(ps);
+        P->axis_tw[0].n = (int)nx;
+        P->axis_tw[0].e = 16;
+        P->axis_tw[0].table = tw;
+    } else if (shape == SHAPE_BATCH) {
+        rc = pl.axis(nx, Geo{1, 0, ny, nx, 1}, &cur, 0);
+    } else if (shape == 1 && !is_pow2(nx)) {
         // non-power-of-two 1D: GPU DFT with exactly reduced phases
         void *roots = nullptr;
         err = hipMalloc(&roots, (size_t)nx * 16);
@@ -505,6 +591,7 @@ int build_plan(struct topeFFT *f, int ndim, long long nx, long long ny, long lon
         err = gen_power_table(roots, PREC_F64, nx, 1, nx, s);
         if (err != hipSuccess) return fail(map_err(err));
         Pass ps;
+        ps.kind = PASS_DFT;
         ps.dft_n = nx;
         ps.dft_roots = roots;
         ps.in_buf = 0;
@@ -518,8 +605,8 @@ int build_plan(struct topeFFT *f, int ndim, long long nx, long long ny, long lon
     } else {
         // X: rows; Y: columns of each plane; Z: columns of the whole volume
         rc = pl.axis(nx, Geo{1, 0, ny * nz, nx, 1}, &cur, 0);
-        if (!rc && ndim >= 2) rc = pl.axis(ny, Geo{nz, nx * ny, nx, 1, nx}, &cur, 1);
-        if (!rc && ndim >= 3) rc = pl.axis(nz, Geo{1, 0, nx * ny, 1, nx * ny}, &cur, 2);
+        if (!rc && shape >= 2) rc = pl.axis(ny, Geo{nz, nx * ny, nx, 1, nx}, &cur, 1);
+        if (!rc && shape >= 3) rc = pl.axis(nz, Geo{1, 0, nx * ny, 1, nx * ny}, &cur, 2);
     }
     if (rc) return fail(rc);
     if (!P->passes.empty()) P->passes.back().last = true;
@@ -578,7 +665,7 @@ void release_framework(struct topeFFT *f)
 
 // Exec with the reference's semantics: transform, then blocking copy of the
 // result into d (src/fft2d.c:193-198).
-int exec_host(struct topeFFT *f, PlanImpl *P, double *d, int dir, tope_ulong *kern_ns, tope_ulong *mem_ns)
+int exec_host(struct topeFFT *f, PlanImpl *P, void *d, int dir, tope_ulong *kern_ns, tope_ulong *mem_ns)
 {
     FrameworkImpl *fw = fw_of(f);
     if (!fw || P->fw != fw) return TOPE_ERROR_NOT_INITIALIZED;
@@ -686,7 +773,7 @@ void tope1DPlanInit(struct topeFFT *f, struct topePlan1D *t, int x, int type, do
     t->dim = 1;
     ref_radix(x, &t->radix, &t->log);
     PlanImpl *P = nullptr;
-    int rc = build_plan(f, 1, x, 1, 1, d, &P, &t->totalPreKernel, &t->totalMemory);
+    int rc = build_plan(f, 1, x, 1, 1, PREC_F64, MP_VALU, d, &P, &t->totalPreKernel, &t->totalMemory);
     f->error = rc;
     if (rc) return;
     P->fw->live_plans++;
@@ -740,7 +827,7 @@ void tope2DPlanInit(struct topeFFT *f, struct topePlan2D *t, int x, int y, int t
     ref_radix(x, &t->radX, &t->logX);
     ref_radix(y, &t->radY, &t->logY);
     PlanImpl *P = nullptr;
-    int rc = build_plan(f, 2, x, y, 1, d, &P, &t->totalPreKernel, &t->totalMemory);
+    int rc = build_plan(f, 2, x, y, 1, PREC_F64, MP_VALU, d, &P, &t->totalPreKernel, &t->totalMemory);
     f->error = rc;
     if (rc) return;
     P->fw->live_plans++;
@@ -797,7 +884,7 @@ void tope3DPlanInit(struct topeFFT *f, struct topePlan3D *t, int x, int y, int z
     ref_radix(y, &t->radY, &t->logY);
     ref_radix(z, &t->radZ, &t->logZ);
     PlanImpl *P = nullptr;
-    int rc = build_plan(f, 3, x, y, z, d, &P, &t->totalPreKernel, &t->totalMemory);
+    int rc = build_plan(f, 3, x, y, z, PREC_F64, MP_VALU, d, &P, &t->totalPreKernel, &t->totalMemory);
     f->error = rc;
     if (rc) return;
     P->fw->live_plans++;
@@ -837,6 +924,59 @@ void tope3DDestroy(struct topeFFT *f, struct topePlan3D *t)
     destroy_common(f, impl_of(t->impl));
     t->impl = nullptr;
     t->data = t->scratch = t->twdX = t->twdY = t->twdZ = nullptr;
+    f->error = TOPE_SUCCESS;
+}
+
+/* ---- batched 1D transforms ---------------------------------------------- */
+
+void topeBatchPlanInit(struct topeFFT *f, struct topePlanBatch *t, int n, int count, int precision, int mode,
+                       const void *host)
+{
+    if (!f || !t) return;
+    std::memset(t, 0, sizeof(*t));
+    t->n = n;
+    t->count = count;
+    t->precision = precision;
+    t->mode = mode;
+    PlanImpl *P = nullptr;
+    int rc = build_plan(f, SHAPE_BATCH, n, count, 1, precision, mode, host, &P, &t->totalPreKernel,
+                        &t->totalMemory);
+    f->error = rc;
+    if (rc) return;
+    P->fw->live_plans++;
+    t->impl = P;
+    t->dataSize = P->bytes;
+    t->data = P->buf[0];
+}
+
+void topeBatchExec(struct topeFFT *f, struct topePlanBatch *t, void *host, int dir)
+{
+    if (!f || !t || !t->impl) {
+        if (f) f->error = TOPE_ERROR_INVALID_ARG;
+        return;
+    }
+    PlanImpl *P = impl_of(t->impl);
+    f->error = exec_host(f, P, host, dir, &t->totalKernel, &t->totalMemory);
+    t->data = P->buf[0];
+}
+
+void topeBatchExecDevice(struct topeFFT *f, struct topePlanBatch *t, int dir)
+{
+    if (!f || !t || !t->impl) {
+        if (f) f->error = TOPE_ERROR_INVALID_ARG;
+        return;
+    }
+    PlanImpl *P = impl_of(t->impl);
+    f->error = exec_device(f, P, dir);
+    t->data = P->buf[0];
+}
+
+void topeBatchDestroy(struct topeFFT *f, struct topePlanBatch *t)
+{
+    if (!f || !t || !t->impl) return;
+    destroy_common(f, impl_of(t->impl));
+    t->impl = nullptr;
+    t->data = nullptr;
     f->error = TOPE_SUCCESS;
 }
 
@@ -926,6 +1066,32 @@ int topePlanPassTimes(struct topeFFT *f, void *plan_impl, double *ms, int max_pa
     return n;
 }
 
+double topePlanTimePass(struct topeFFT *f, void *plan_impl, int pass, int dir, int reps)
+{
+    PlanImpl *P = impl_of(plan_impl);
+    FrameworkImpl *fw = fw_of(f);
+    if (!P || !fw || pass < 0 || pass >= (int)P->passes.size() || reps < 1 ||
+        (dir != FORWARD && dir != INVERSE)) {
+        if (f) f->error = TOPE_ERROR_INVALID_ARG;
+        return -1.0;
+    }
+    const Pass &ps = P->passes[pass];
+    if (ps.in_buf != ps.out_buf) {  // out-of-place steps cannot be repeated in isolation
+        f->error = TOPE_ERROR_UNSUPPORTED;
+        return -1.0;
+    }
+    EventPair ev;
+    hipError_t err = time_start(ev, fw->stream);
+    int rc = TOPE_SUCCESS;
+    for (int r = 0; r < reps && err == hipSuccess && !rc; ++r) rc = launch_pass(P, pass, fw->stream, dir);
+    tope_ulong ns = 0;
+    if (err == hipSuccess && !rc) err = time_stop(ev, fw->stream, &ns);
+    if (ev.start) (void)hipEventDestroy(ev.start);
+    if (ev.stop) (void)hipEventDestroy(ev.stop);
+    f->error = rc ? rc : map_err(err);
+    return f->error ? -1.0 : (double)ns * 1e-6 / reps;
+}
+
 const char *topePlanPassName(void *plan_impl, int pass)
 {
     PlanImpl *P = impl_of(plan_impl);
@@ -940,6 +1106,7 @@ size_t topeStructSize(int which)
     case 1: return sizeof(struct topePlan1D);
     case 2: return sizeof(struct topePlan2D);
     case 3: return sizeof(struct topePlan3D);
+    case 4: return sizeof(struct topePlanBatch);
     default: return 0;
     }
 }
@@ -970,7 +1137,7 @@ long long topePlanTwiddles(struct topeFFT *f, void *plan_impl, int axis, double 
     long long count = at.e ? host_sched(at.n, at.e).twsize : at.n;
     if (!host) return count;
     long long n = count < max_entries ? count : max_entries;
-    hipError_t err = hipMemcpyAsync(host, at.table, (size_t)n * 16, hipMemcpyDeviceToHost, fw->stream);
+    hipError_t err = hipMemcpyAsync(host, at.table, (size_t)n * P->elem, hipMemcpyDeviceToHost, fw->stream);
     if (err == hipSuccess) err = hipStreamSynchronize(fw->stream);
     f->error = map_err(err);
     return err == hipSuccess ? n : -1;

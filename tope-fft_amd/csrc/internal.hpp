@@ -55,4 +55,11 @@ hipError_t gen_power_table(void *out, int prec, long long M, long long mult, lon
 hipError_t launch_naive_dft(int prec, const void *in, void *out, const void *roots, long long n, int sign,
                             double scale, hipStream_t s);
 
+// Mixed-precision MFMA rows (mfma_kernel.hpp).
+struct MfmaArgs;
+hipError_t gen_mfma_constants(void *out, hipStream_t s);
+size_t mfma_constants_bytes();
+bool mfma_supported(int n);
+hipError_t launch_mfma_rows(int mode, int n, const void *in, void *out, const MfmaArgs &a, int sign, hipStream_t s);
+
 }  // namespace tope

@@ -82,7 +82,7 @@ constexpr int prec_of() { return sizeof(R) == 8 ? PREC_F64 : PREC_F32; }
         VAR(R, 4096, 32, 2, true, false), VAR(R, 4096, 32, 4, true, true), VAR(R, 2048, 32, 4, true, false), \
         VAR(R, 2048, 32, 8, true, true)
 
-static const AxisVariant kVariants[] = {ROWS(double), COLS(double)};
+static const AxisVariant kVariants[] = {ROWS(double), COLS(double), ROWS(float), COLS(float)};
 
 // The same list is compiled twice: normally (32-bit buffer offsets) and with
 // TOPE_AXIS_BIG (64-bit pointers) for blocks spanning more than 2^31 bytes.

@@ -54,9 +54,21 @@ class TopePlan3D(C.Structure):
                 ("twdX", C.c_void_p), ("twdY", C.c_void_p), ("twdZ", C.c_void_p)] + _COMMON_TAIL
 
 
+class TopePlanBatch(C.Structure):
+    _fields_ = [("n", C.c_int), ("count", C.c_int), ("precision", C.c_int), ("mode", C.c_int),
+                ("data", C.c_void_p), ("dataSize", C.c_size_t), ("totalMemory", C.c_uint64),
+                ("totalPreKernel", C.c_uint64), ("totalKernel", C.c_uint64), ("impl", C.c_void_p)]
+
+
+PREC_F64, PREC_F32 = 0, 1
+MODE_VALU, MODE_MFMA_BF16X3, MODE_MFMA_BF16X6, MODE_MFMA_FP32 = 0, 1, 2, 3
+MODE_NAMES = {MODE_VALU: "valu", MODE_MFMA_BF16X3: "mfma_bf16x3", MODE_MFMA_BF16X6: "mfma_bf16x6",
+              MODE_MFMA_FP32: "mfma_fp32"}
+
 # name -> (restype, argtypes)
 _F, _P1, _P2, _P3 = C.POINTER(TopeFFT), C.POINTER(TopePlan1D), C.POINTER(TopePlan2D), C.POINTER(TopePlan3D)
 _DP = C.POINTER(C.c_double)
+_PB = C.POINTER(TopePlanBatch)
 SIGNATURES = {
     "topeFFTInit": (None, [_F]),
     "topeFFTDestroy": (None, [_F]),
@@ -80,7 +92,12 @@ SIGNATURES = {
     "topePlanPassTiming": (None, [C.c_void_p, C.c_int]),
     "topePlanPassTimes": (C.c_int, [_F, C.c_void_p, _DP, C.c_int]),
     "topePlanPassName": (C.c_char_p, [C.c_void_p, C.c_int]),
+    "topePlanTimePass": (C.c_double, [_F, C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "topeStructSize": (C.c_size_t, [C.c_int]),
+    "topeBatchPlanInit": (None, [_F, _PB, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]),
+    "topeBatchExec": (None, [_F, _PB, C.c_void_p, C.c_int]),
+    "topeBatchExecDevice": (None, [_F, _PB, C.c_int]),
+    "topeBatchDestroy": (None, [_F, _PB]),
     "topeErrorString": (C.c_char_p, [C.c_int]),
     "topePlanTwiddles": (C.c_longlong, [_F, C.c_void_p, C.c_int, _DP, C.c_longlong]),
 }
@@ -105,8 +122,9 @@ def load(path=LIB_PATH):
 
 def check_layout():
     lib = load()
-    mine = [C.sizeof(TopeFFT), C.sizeof(TopePlan1D), C.sizeof(TopePlan2D), C.sizeof(TopePlan3D)]
-    theirs = [lib.topeStructSize(i) for i in range(4)]
+    mine = [C.sizeof(TopeFFT), C.sizeof(TopePlan1D), C.sizeof(TopePlan2D), C.sizeof(TopePlan3D),
+            C.sizeof(TopePlanBatch)]
+    theirs = [lib.topeStructSize(i) for i in range(5)]
     return mine, theirs
 
 
@@ -220,6 +238,11 @@ class Plan:
         self.fw.check("topePlanPassTimes")
         return ms[:n]
 
+    def time_pass(self, i, direction=FORWARD, reps=20):
+        ms = self.lib.topePlanTimePass(C.byref(self.fw.f), self.impl, i, direction, reps)
+        self.fw.check("topePlanTimePass")
+        return ms
+
     def twiddles(self, axis):
         cnt = self.lib.topePlanTwiddles(C.byref(self.fw.f), self.impl, axis, None, 0)
         if cnt < 0:
@@ -233,6 +256,53 @@ class Plan:
         if self.t.impl:
             fn = getattr(self.lib, f"tope{self.ndim}DDestroy")
             fn(C.byref(self.fw.f), C.byref(self.t))
+
+
+class BatchPlan:
+    """`count` length-`n` transforms (topeBatchPlanInit).  FP64 plans take
+    complex128 input, FP32 plans complex64."""
+
+    def __init__(self, fw, n, count, x, precision=PREC_F64, mode=MODE_VALU):
+        self.fw, self.lib = fw, fw.lib
+        self.n, self.count, self.precision, self.mode = n, count, precision, mode
+        self.dtype = np.complex128 if precision == PREC_F64 else np.complex64
+        host = np.ascontiguousarray(x, dtype=self.dtype).reshape(-1)
+        if host.size != n * count:
+            raise ValueError("input size does not match n * count")
+        self.host = host
+        self.t = TopePlanBatch()
+        self.lib.topeBatchPlanInit(C.byref(fw.f), C.byref(self.t), n, count, precision, mode,
+                                   host.ctypes.data_as(C.c_void_p))
+        fw.check(f"topeBatchPlanInit(n={n}, count={count}, prec={precision}, mode={mode})")
+
+    @property
+    def impl(self):
+        return self.t.impl
+
+    def exec(self, direction=FORWARD):
+        out = np.empty(self.n * self.count, dtype=self.dtype)
+        self.lib.topeBatchExec(C.byref(self.fw.f), C.byref(self.t), out.ctypes.data_as(C.c_void_p), direction)
+        self.fw.check("topeBatchExec")
+        return out
+
+    def exec_device(self, direction=FORWARD):
+        self.lib.topeBatchExecDevice(C.byref(self.fw.f), C.byref(self.t), direction)
+        self.fw.check("topeBatchExecDevice")
+
+    def pass_count(self):
+        return self.lib.topePlanPassCount(self.impl)
+
+    def pass_names(self):
+        return [self.lib.topePlanPassName(self.impl, i).decode() for i in range(self.pass_count())]
+
+    def time_pass(self, i, direction=FORWARD, reps=20):
+        ms = self.lib.topePlanTimePass(C.byref(self.fw.f), self.impl, i, direction, reps)
+        self.fw.check("topePlanTimePass")
+        return ms
+
+    def destroy(self):
+        if self.t.impl:
+            self.lib.topeBatchDestroy(C.byref(self.fw.f), C.byref(self.t))
 
 
 def transform(x, dims, direction=FORWARD):
