@@ -248,3 +248,22 @@ def test_destroy_last_plan_releases_framework():
     with pytest.raises(topefft.TopeError) as e:
         topefft.Plan(f, (8,), np.ones(8, complex))
     assert e.value.code == topefft.TOPE_ERROR_NOT_INITIALIZED
+
+
+@pytest.mark.parametrize("dims", [(8, 4), (16, 64), (64, 32), (256, 256), (1024, 512), (2, 16), (4096, 16)],
+                         ids=lambda d: "x".join(map(str, d)))
+def test_2d_dif2_schedule_small(oracle, monkeypatch, dims):
+    # the schedule 2D 4096^2 uses (X pass on row pairs with the radix-2 DIF
+    # step of Y, then half-length Y columns written interleaved), forced on
+    # small shapes where the oracle's naive DFT is cheap
+    monkeypatch.setenv("TOPEFFT_DIF2_MIN", "4")
+    x = seeded(dims, salt=3)
+    fw = topefft.Framework()
+    p = topefft.Plan(fw, dims, x)
+    names = p.pass_names()
+    assert "dif2" in names[0], names
+    X = p.exec(topefft.FORWARD)
+    back = p.exec(topefft.INVERSE)
+    p.destroy()
+    assert rel_l2(X, oracle.dft(x, dims, 1)) < TOL
+    assert rel_l2(back, x) < TOL

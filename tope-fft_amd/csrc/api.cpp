@@ -67,6 +67,7 @@ struct PlanImpl {
     int ndim = 0;
     long long dims[3] = {1, 1, 1};
     long long total = 0;
+    long long norm = 0;  // elements of one transform: INVERSE divides by it
     size_t bytes = 0;
     void *buf[2] = {nullptr, nullptr};
     int final_buf = 0;
@@ -149,6 +150,7 @@ const AxisVariant *find_variant_named(const char *name)
 const char *const kPreferred[] = {
     "axis_double_n4096_e16_tb1_row_split",
     "axis_double_n4096_e16_tb4_col_split",
+    "axis_double_n2048_e16_tb4_col_split",
 };
 
 // Variant choice: a measured preference if the line count fills its tile;
@@ -156,24 +158,24 @@ const char *const kPreferred[] = {
 // (wider tiles give longer contiguous segments per row), preferring the
 // full-complex exchange at equal width.  TOPEFFT_VARIANT_<n>_<row|col>=<name>
 // overrides (used by tuning sweeps).
-const AxisVariant *select_variant(int prec, int n, bool cfast, long long lines)
+const AxisVariant *select_variant(int prec, int n, bool cfast, long long lines, bool dif2 = false)
 {
     char key[64];
-    std::snprintf(key, sizeof key, "TOPEFFT_VARIANT_%d_%s", n, cfast ? "col" : "row");
+    std::snprintf(key, sizeof key, "TOPEFFT_VARIANT_%d_%s%s", n, cfast ? "col" : "row", dif2 ? "_dif2" : "");
     if (const char *ov = std::getenv(key)) {
         const AxisVariant *v = find_variant_named(ov);
-        if (v && v->prec == prec && v->n == n) return v;
+        if (v && v->prec == prec && v->n == n && v->dif2 == dif2) return v;
     }
     for (const char *name : kPreferred) {
         const AxisVariant *v = find_variant_named(name);
-        if (v && v->prec == prec && v->n == n && v->cfast == cfast && v->tb <= lines) return v;
+        if (v && v->prec == prec && v->n == n && v->cfast == cfast && v->dif2 == dif2 && v->tb <= lines) return v;
     }
     int cnt = 0;
     const AxisVariant *vs = axis_variants(&cnt);
     const AxisVariant *best = nullptr;
     for (int i = 0; i < cnt; ++i) {
         const AxisVariant *v = &vs[i];
-        if (v->prec != prec || v->n != n || v->cfast != cfast) continue;
+        if (v->prec != prec || v->n != n || v->cfast != cfast || v->dif2 != dif2) continue;
         if (!best) {
             best = v;
             continue;
@@ -279,10 +281,12 @@ class Planner {
     // bi, output geometry go in buffer bo.
     int add_pass(int n, const Geo &gi, const Geo &go, int bi, int bo, long long reps, long long rep_in,
                  long long rep_out, const void *pt_lo = nullptr, const void *pt_hi = nullptr, long long pt_o = 0,
-                 long long pt_c = 0, long long pt_n = 0, int pt_shift = 0, int axis = -1)
+                 long long pt_c = 0, long long pt_n = 0, int pt_shift = 0, int axis = -1, long long pair = 0,
+                 const void *dif_tw = nullptr)
     {
+        const bool dif2 = pair != 0;
         const bool cfast = !(gi.is == 1 && go.is == 1);
-        const AxisVariant *v = select_variant(P->prec, n, cfast, gi.C);
+        const AxisVariant *v = select_variant(P->prec, n, cfast, dif2 ? 2 * gi.C : gi.C, dif2);
         if (!v) return TOPE_ERROR_NO_KERNEL;
         HostSched h = host_sched(v->n, v->e);
         if (h.twsize != v->twsize) return TOPE_ERROR_NO_KERNEL;  // host/device layouts disagree
@@ -305,8 +309,11 @@ class Planner {
         a.out_os = go.os;
         a.out_cs = go.cs;
         a.out_is = go.is;
+        const int rows_per_tile = dif2 ? v->tb / 2 : v->tb;
         a.C = (int)gi.C;
-        a.tiles = (int)((gi.C + v->tb - 1) / v->tb);
+        a.tiles = (int)((gi.C + rows_per_tile - 1) / rows_per_tile);
+        a.pair = pair;
+        a.dif_tw = static_cast<const cpx<double> *>(dif_tw);
         a.tw = static_cast<const cpx<double> *>(tw);
         a.pt_lo = static_cast<const cpx<double> *>(pt_lo);
         a.pt_hi = static_cast<const cpx<double> *>(pt_hi);
@@ -319,8 +326,10 @@ class Planner {
         a.in_elems = a.out_elems = P->total;
         {
             // largest byte offset inside one block's footprint, in and out
-            const long long span_in = ((long long)(v->tb - 1) * gi.cs + (long long)(n - 1) * gi.is + 1) * (long long)P->elem;
-            const long long span_out = ((long long)(v->tb - 1) * go.cs + (long long)(n - 1) * go.is + 1) * (long long)P->elem;
+            const long long span_in =
+                ((long long)(rows_per_tile - 1) * gi.cs + pair + (long long)(n - 1) * gi.is + 1) * (long long)P->elem;
+            const long long span_out =
+                ((long long)(rows_per_tile - 1) * go.cs + pair + (long long)(n - 1) * go.is + 1) * (long long)P->elem;
             if (span_in >= 0x7fffffffLL || span_out >= 0x7fffffffLL) {
                 int cnt = 0, cntb = 0;
                 const AxisVariant *base = axis_variants(&cnt);
@@ -349,6 +358,8 @@ class Planner {
             b.scale = 1.0f;
             b.xcd = a.xcd;
             b.in_elems = b.out_elems = P->total;
+            b.pair = pair;
+            b.dif_tw = static_cast<const cpx<float> *>(dif_tw);
         }
         const long long grid = gi.O * (long long)a.tiles;
         if (grid <= 0 || grid > 0x7fffffffLL) return TOPE_ERROR_INVALID_SIZE;
@@ -400,10 +411,40 @@ class Planner {
         return TOPE_SUCCESS;
     }
 
+    // 2D with a long Y axis (ny <= 4096): the X pass takes rows y and
+    // y + ny/2 together and ends with the radix-2 DIF step of Y between
+    // them; the Y pass then runs two independent ny/2-point transforms per
+    // column (rows 0..ny/2-1 -> even outputs, the rest -> odd outputs) and
+    // writes them interleaved into natural order, out of place.  Per-block
+    // column tiles are half as large, so two blocks share a CU and overlap
+    // their memory and compute phases.
+    int plan_2d_dif2(long long nx, long long ny, int *cur)
+    {
+        const long long half = ny / 2;
+        void *dtw = nullptr;
+        hipError_t err = power_table(ny, 1, half, &dtw);
+        if (err != hipSuccess) return map_err(err);
+        const Geo rows{1, 0, half, nx, 1};
+        int rc = add_pass((int)nx, rows, rows, *cur, *cur, 1, 0, 0, nullptr, nullptr, 0, 0, 0, 0, 0, half * nx, dtw);
+        if (rc) return rc;
+        const Geo gin{2, half * nx, nx, 1, nx}, gout{2, nx, nx, 1, 2 * nx};
+        rc = add_pass((int)half, gin, gout, *cur, *cur ^ 1, 1, 0, 0, nullptr, nullptr, 0, 0, 0, 0, 1);
+        if (rc) return rc;
+        *cur ^= 1;
+        return TOPE_SUCCESS;
+    }
+
    private:
     PlanImpl *P;
     hipStream_t st;
 };
+
+// 2D plans whose Y axis is this long (and fits one pass) use plan_2d_dif2.
+long long dif2_min_rows()
+{
+    if (const char *e = std::getenv("TOPEFFT_DIF2_MIN")) return std::atoll(e);
+    return 4096;
+}
 
 hipError_t time_start(EventPair &ev, hipStream_t s)
 {
@@ -443,7 +484,7 @@ hipEvent_t pool_event(PlanImpl *P)
 int launch_pass(PlanImpl *P, size_t i, hipStream_t s, int dir)
 {
     const int sign = (dir == FORWARD) ? -1 : 1;
-    const double sc = (dir == INVERSE) ? 1.0 / (double)P->total : 1.0;
+    const double sc = (dir == INVERSE) ? 1.0 / (double)P->norm : 1.0;
     const Pass &ps = P->passes[i];
     const char *ib = static_cast<const char *>(P->buf[ps.in_buf]);
     char *ob = static_cast<char *>(P->buf[ps.out_buf]);
@@ -537,6 +578,7 @@ int build_plan(struct topeFFT *f, int shape, long long nx, long long ny, long lo
     P->dims[1] = ny;
     P->dims[2] = nz;
     P->total = total;
+    P->norm = shape == SHAPE_BATCH ? nx : total;
     P->bytes = (size_t)total * P->elem;
     hipStream_t s = fw->stream;
 
@@ -602,6 +644,8 @@ int build_plan(struct topeFFT *f, int shape, long long nx, long long ny, long lo
         P->axis_tw[0].n = (int)nx;
         P->axis_tw[0].e = 0;
         P->axis_tw[0].table = roots;
+    } else if (shape == 2 && nx <= kMaxFused && ny <= kMaxFused && ny >= 4 && ny >= dif2_min_rows()) {
+        rc = pl.plan_2d_dif2(nx, ny, &cur);
     } else {
         // X: rows; Y: columns of each plane; Z: columns of the whole volume
         rc = pl.axis(nx, Geo{1, 0, ny * nz, nx, 1}, &cur, 0);

@@ -74,6 +74,13 @@ struct AxisArgs {
     R scale;    // multiplies the output (1/N on the last inverse pass, else 1)
     int xcd;    // 1: remap blocks so each XCD walks a contiguous range of tiles
     long long in_elems, out_elems;  // buffer sizes (bounds of the buffer resources)
+    // DIF2 row passes: block-local transform t = 2q + h is row (tile*TB/2 + q)
+    // shifted by h*pair elements; afterwards the two rows of each pair are
+    // combined by one radix-2 decimation-in-frequency step of the NEXT axis
+    // (length 2*half_rows): row h=0 <- r0 + r1, row h=1 <- (r0 - r1) W^{row},
+    // W = W_{2*half_rows} read from dif_tw (forward sign).
+    long long pair;
+    const cpx<R> *dif_tw;
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
@@ -272,13 +279,14 @@ __device__ __forceinline__ void buf_store(cpx<R> v, __amdgpu_buffer_rsrc_t r, in
 // stride e*P*is*16 is a uniform (SGPR) offset, which keeps address
 // registers out of the way of the 4*E data registers.  Blocks whose
 // footprint exceeds 2^31 bytes run the BIG instantiation (64-bit pointers).
-template <typename R, int N, int E, int TB, bool CFAST, bool SPLIT, int SIGN, bool POST, bool BIG>
+template <typename R, int N, int E, int TB, bool CFAST, bool SPLIT, int SIGN, bool POST, bool BIG, bool DIF2 = false>
 __global__ void __launch_bounds__(TB *(N / E))
 axis_kernel(const cpx<R> *in, cpx<R> *out, const AxisArgs<R> a)
 {
     using SC = Sched<N, E>;
     constexpr int P = SC::P;
-    constexpr bool USE_LDS = (P > 1) && (SC::NST > 1);
+    static_assert(!DIF2 || (!CFAST && TB % 2 == 0), "DIF2 pairs rows within a row block");
+    constexpr bool USE_LDS = DIF2 || ((P > 1) && (SC::NST > 1));
     constexpr int LDS_BYTES = USE_LDS ? Exchange<R, N, E, TB, CFAST, SPLIT>::BYTES : 16;
     constexpr int CB = (int)sizeof(cpx<R>);
     __shared__ __align__(16) unsigned char lds[LDS_BYTES];
@@ -289,15 +297,19 @@ axis_kernel(const cpx<R> *in, cpx<R> *out, const AxisArgs<R> a)
     const long long blk = xcd_block(a.xcd);
     const long long o = blk / a.tiles;
     const int tile = (int)(blk - o * a.tiles);
-    const int c0 = tile * TB;
-    const int c = c0 + t;
+    // DIF2: TB/2 row pairs per block, transform t = 2q + h
+    constexpr int TROWS = DIF2 ? TB / 2 : TB;
+    const int c0 = tile * TROWS;
+    const int c = c0 + (DIF2 ? (t >> 1) : t);
+    const long long hoff = DIF2 ? (long long)(t & 1) * a.pair : 0;
+    const int tr = DIF2 ? (t >> 1) : t;  // row within the tile
     const bool valid = c < a.C;
 
     cpx<R> v[E];
     if constexpr (!BIG) {
         const long long ibase = o * a.in_os + (long long)c0 * a.in_cs;
         const __amdgpu_buffer_rsrc_t rin = make_rsrc(in + ibase, (a.in_elems - ibase) * CB);
-        const int voff = (int)((t * a.in_cs + (long long)p * a.in_is) * CB);
+        const int voff = (int)((tr * a.in_cs + hoff + (long long)p * a.in_is) * CB);
         const int sstep = (int)((long long)P * a.in_is * CB);
         if (valid) {
 #pragma unroll
@@ -308,7 +320,7 @@ axis_kernel(const cpx<R> *in, cpx<R> *out, const AxisArgs<R> a)
         }
     } else {
         if (valid) {
-            const cpx<R> *src = in + o * a.in_os + (long long)c * a.in_cs;
+            const cpx<R> *src = in + o * a.in_os + (long long)c * a.in_cs + hoff;
 #pragma unroll
             for (int e = 0; e < E; ++e) v[e] = src[(long long)(p + P * e) * a.in_is];
         } else {
@@ -318,6 +330,32 @@ axis_kernel(const cpx<R> *in, cpx<R> *out, const AxisArgs<R> a)
     }
 
     run_stages<R, N, E, TB, CFAST, SPLIT, SIGN, 0>(v, p, t, a.tw, lds);
+
+    if constexpr (DIF2) {
+        // radix-2 DIF step of the next axis between the rows of each pair,
+        // exchanged through LDS one component at a time (no extra registers)
+        R *l = (R *)lds;
+        const int mate = t ^ 1;
+        const int rbase = lds_index<N, TB, CFAST>(t, p), mbase = lds_index<N, TB, CFAST>(mate, p);
+#pragma unroll
+        for (int comp = 0; comp < 2; ++comp) {
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < E; ++e) l[rbase + lds_off<TB, CFAST>(P * e)] = comp ? v[e].y : v[e].x;
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const R m = l[mbase + lds_off<TB, CFAST>(P * e)];
+                R &mine = comp ? v[e].y : v[e].x;
+                mine = (t & 1) ? (m - mine) : (mine + m);
+            }
+        }
+        if (t & 1) {
+            const cpx<R> w = a.dif_tw[c < a.C ? c : 0];
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e] = twmul<SIGN>(v[e], w);
+        }
+    }
 
     if (!valid) return;
     if constexpr (POST) {
@@ -336,12 +374,12 @@ axis_kernel(const cpx<R> *in, cpx<R> *out, const AxisArgs<R> a)
     if constexpr (!BIG) {
         const long long obase = o * a.out_os + (long long)c0 * a.out_cs;
         const __amdgpu_buffer_rsrc_t rout = make_rsrc(out + obase, (a.out_elems - obase) * CB);
-        const int voff = (int)((t * a.out_cs + (long long)p * a.out_is) * CB);
+        const int voff = (int)((tr * a.out_cs + hoff + (long long)p * a.out_is) * CB);
         const int sstep = (int)((long long)P * a.out_is * CB);
 #pragma unroll
         for (int e = 0; e < E; ++e) buf_store<R>(v[e], rout, voff, e * sstep);
     } else {
-        cpx<R> *dst = out + o * a.out_os + (long long)c * a.out_cs;
+        cpx<R> *dst = out + o * a.out_os + (long long)c * a.out_cs + hoff;
 #pragma unroll
         for (int e = 0; e < E; ++e) dst[(long long)(p + P * e) * a.out_is] = v[e];
     }

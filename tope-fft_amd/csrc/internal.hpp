@@ -21,6 +21,7 @@ struct AxisVariant {
     int tb;       // transforms per block
     bool cfast;   // lanes walk transforms first (strided-axis tiles)
     bool split;   // real/imag LDS exchange in two rounds
+    bool dif2;    // row pairs + radix-2 DIF step of the next axis
     int threads;  // block size
     int twsize;   // stage twiddle entries for (n, e)
     int lds_bytes;

@@ -11,7 +11,7 @@
 
 namespace tope {
 
-template <typename R, int N, int E, int TB, bool CFAST, bool SPLIT, bool BIG>
+template <typename R, int N, int E, int TB, bool CFAST, bool SPLIT, bool BIG, bool DIF2>
 static void launch_axis_t(const void *in, void *out, const void *args, unsigned grid, hipStream_t s, int sign)
 {
     constexpr int NT = TB * (N / E);
@@ -31,17 +31,17 @@ static void launch_axis_t(const void *in, void *out, const void *args, unsigned 
         }
     }
     if (sign < 0)
-        hipLaunchKernelGGL((axis_kernel<R, N, E, TB, CFAST, SPLIT, -1, false, BIG>), dim3(grid), dim3(NT), 0, s, i,
-                           o, a);
+        hipLaunchKernelGGL((axis_kernel<R, N, E, TB, CFAST, SPLIT, -1, false, BIG, DIF2>), dim3(grid), dim3(NT), 0, s,
+                           i, o, a);
     else
-        hipLaunchKernelGGL((axis_kernel<R, N, E, TB, CFAST, SPLIT, 1, false, BIG>), dim3(grid), dim3(NT), 0, s, i,
-                           o, a);
+        hipLaunchKernelGGL((axis_kernel<R, N, E, TB, CFAST, SPLIT, 1, false, BIG, DIF2>), dim3(grid), dim3(NT), 0, s,
+                           i, o, a);
 }
 
 #ifdef TOPE_AXIS_BIG
-#define TOPE_LAUNCH(R, N, E, TB, CF, SP) &launch_axis_t<R, N, E, TB, CF, SP, true>
+#define TOPE_LAUNCH(R, N, E, TB, CF, SP, DF) &launch_axis_t<R, N, E, TB, CF, SP, true, DF>
 #else
-#define TOPE_LAUNCH(R, N, E, TB, CF, SP) &launch_axis_t<R, N, E, TB, CF, SP, false>
+#define TOPE_LAUNCH(R, N, E, TB, CF, SP, DF) &launch_axis_t<R, N, E, TB, CF, SP, false, DF>
 #endif
 
 template <typename R>
@@ -51,13 +51,26 @@ constexpr int prec_of() { return sizeof(R) == 8 ? PREC_F64 : PREC_F32; }
 #define TOPE_LAYOUT_false "_row"
 #define TOPE_SPLIT_true "_split"
 #define TOPE_SPLIT_false ""
+#define TOPE_DIF2_true "_dif2"
+#define TOPE_DIF2_false ""
 #define TOPE_STR2(x) #x
 #define TOPE_STR(x) TOPE_STR2(x)
-#define VAR(R, N, E, TB, CF, SP)                                                                             \
-    AxisVariant{prec_of<R>(), N, E, TB, CF, SP, TB * (N / E), Sched<N, E>::TWSIZE,                          \
-                Exchange<R, N, E, TB, CF, SP>::BYTES, TOPE_LAUNCH(R, N, E, TB, CF, SP),                    \
+#define VARD(R, N, E, TB, CF, SP, DF)                                                                        \
+    AxisVariant{prec_of<R>(), N, E, TB, CF, SP, DF, TB * (N / E), Sched<N, E>::TWSIZE,                      \
+                Exchange<R, N, E, TB, CF, SP>::BYTES, TOPE_LAUNCH(R, N, E, TB, CF, SP, DF),                \
                 "axis_" TOPE_STR(R) "_n" TOPE_STR(N) "_e" TOPE_STR(E) "_tb" TOPE_STR(TB) TOPE_LAYOUT_##CF \
-                    TOPE_SPLIT_##SP}
+                    TOPE_SPLIT_##SP TOPE_DIF2_##DF}
+#define VAR(R, N, E, TB, CF, SP) VARD(R, N, E, TB, CF, SP, false)
+
+// Row-pair variants (2D passes whose next axis takes a fused radix-2 DIF
+// step): the row table's tiles, at least two rows, split exchange.
+#define DIF2ROWS(R)                                                                                              \
+    VARD(R, 2, 2, 256, false, true, true), VARD(R, 4, 4, 256, false, true, true),                               \
+        VARD(R, 8, 8, 256, false, true, true), VARD(R, 16, 16, 256, false, true, true),                         \
+        VARD(R, 32, 16, 128, false, true, true), VARD(R, 64, 16, 64, false, true, true),                        \
+        VARD(R, 128, 16, 32, false, true, true), VARD(R, 256, 16, 16, false, true, true),                       \
+        VARD(R, 512, 16, 8, false, true, true), VARD(R, 1024, 16, 4, false, true, true),                        \
+        VARD(R, 2048, 16, 2, false, true, true), VARD(R, 4096, 16, 2, false, true, true)
 
 // Row variants: 256 threads per block.
 #define ROWS(R)                                                                                     \
@@ -82,7 +95,7 @@ constexpr int prec_of() { return sizeof(R) == 8 ? PREC_F64 : PREC_F32; }
         VAR(R, 4096, 32, 2, true, false), VAR(R, 4096, 32, 4, true, true), VAR(R, 2048, 32, 4, true, false), \
         VAR(R, 2048, 32, 8, true, true)
 
-static const AxisVariant kVariants[] = {ROWS(double), COLS(double), ROWS(float), COLS(float)};
+static const AxisVariant kVariants[] = {ROWS(double), COLS(double), DIF2ROWS(double), ROWS(float), COLS(float)};
 
 // The same list is compiled twice: normally (32-bit buffer offsets) and with
 // TOPE_AXIS_BIG (64-bit pointers) for blocks spanning more than 2^31 bytes.

@@ -138,22 +138,22 @@ __device__ __forceinline__ void dft16_mfma(const float (&x)[8], cpx<float> (&y)[
 template <int N>
 __device__ __forceinline__ int mp_lds(int t, int i) { return t * (N + N / 16) + i + (i >> 4); }
 
-// One block = TPB transforms of length N = 16^NST, WPT waves per transform,
-// GPW groups of 16 butterflies per wave per stage.
+// One block = 8 waves: TPB transforms of length N = 16^NST, WPT waves per
+// transform, GPW groups of 16 butterflies per wave per stage.
 template <int MODE, int N, int SIGN>
 struct MfmaShape {
     static constexpr int NST = ilog2c(N) / 4;
     static constexpr int NB = N / 16;                    // butterflies per stage
     static constexpr int NG = NB / 16;                   // groups per stage
-    static constexpr int WPT = NG >= 4 ? 4 : NG;         // waves per transform
+    static constexpr int WPT = NG >= 8 ? 8 : NG;         // waves per transform
     static constexpr int GPW = NG / WPT;                 // groups per wave
-    static constexpr int TPB = 4 / WPT;                  // transforms per block (4 waves)
-    static constexpr int THREADS = 256;
+    static constexpr int TPB = 8 / WPT;                  // transforms per block
+    static constexpr int THREADS = 512;
     static_assert(N == (1 << (4 * NST)) && NST >= 2, "N must be 16^s, s >= 2");
 };
 
 template <int MODE, int N, int SIGN>
-__global__ void __launch_bounds__(256) mfma_rows_kernel(const cpx<float> *in, cpx<float> *out, const MfmaArgs a)
+__global__ void __launch_bounds__(512) mfma_rows_kernel(const cpx<float> *in, cpx<float> *out, const MfmaArgs a)
 {
     using SH = MfmaShape<MODE, N, SIGN>;
     constexpr int NB = SH::NB, WPT = SH::WPT, GPW = SH::GPW, TPB = SH::TPB, NST = SH::NST;
@@ -161,7 +161,9 @@ __global__ void __launch_bounds__(256) mfma_rows_kernel(const cpx<float> *in, cp
     __shared__ cpx<float> lds[TPB * (N + N / 16)];
 
     const int tid = threadIdx.x;
-    const int wave = tid >> 6, lane = tid & 63;
+    // wave index made provably uniform: buffer descriptors derived from it
+    // must live in SGPRs (otherwise the compiler emits waterfall loops)
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     const int c = lane & 15, g = lane >> 4;
     const int tb = wave / WPT, wt = wave % WPT;
     const long long row = (long long)blockIdx.x * TPB + tb;
@@ -185,10 +187,16 @@ __global__ void __launch_bounds__(256) mfma_rows_kernel(const cpx<float> *in, cp
         }
     }
 
+    // Per stage: inputs are in registers (from HBM for stage 0, from LDS
+    // after a barrier otherwise); each group's 4 outputs per lane go to LDS
+    // (or HBM in the last stage) right after its product, so accumulators
+    // and outputs never stay live across groups.
 #pragma unroll
     for (int S = 0; S < NST; ++S) {
         const int Ns = 1 << (4 * S);
-        cpx<float> y[GPW][4];
+        const bool last = S == NST - 1;
+        __amdgpu_buffer_rsrc_t rout;
+        if (last) rout = make_rsrc(out + row * N, (a.out_elems - row * N) * CB);
 #pragma unroll
         for (int q = 0; q < GPW; ++q) {
             const int j = (wt * GPW + q) * 16 + c;
@@ -198,37 +206,32 @@ __global__ void __launch_bounds__(256) mfma_rows_kernel(const cpx<float> *in, cp
                 const cpx<float> *t = a.tw + SC::twoff(S) + k;
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
+                    // input m = 0 takes twiddle 1 (its table slot is clamped
+                    // to m = 1 and then replaced, keeping the lanes converged)
                     const int m = 4 * g + s;
-                    if (m == 0) continue;
-                    const cpx<float> w = t[(m - 1) * Ns];
+                    cpx<float> w = t[(m > 0 ? m - 1 : 0) * Ns];
+                    if (m == 0) w = cpx<float>{1.0f, 0.0f};
                     const cpx<float> v = twmul<SIGN>(cpx<float>{x[q][s], x[q][4 + s]}, w);
                     x[q][s] = v.x;
                     x[q][4 + s] = v.y;
                 }
             }
-            dft16_mfma<MODE>(x[q], y[q], a.k, dir, lane);
-        }
-        if (S == NST - 1) {
-            if (valid) {
-                const __amdgpu_buffer_rsrc_t rout = make_rsrc(out + row * N, (a.out_elems - row * N) * CB);
-#pragma unroll
-                for (int q = 0; q < GPW; ++q) {
-                    const int j = (wt * GPW + q) * 16 + c;
+            cpx<float> y[4];
+            dft16_mfma<MODE>(x[q], y, a.k, dir, lane);
+            if (last) {
+                if (valid) {
 #pragma unroll
                     for (int r = 0; r < 4; ++r)
-                        buf_store<float>(scale(y[q][r], a.scale), rout, (j + NB * (4 * g + r)) * CB, 0);
+                        buf_store<float>(scale(y[r], a.scale), rout, (j + NB * (4 * g + r)) * CB, 0);
                 }
-            }
-        } else {
-            if (S > 0) __syncthreads();
-#pragma unroll
-            for (int q = 0; q < GPW; ++q) {
-                const int j = (wt * GPW + q) * 16 + c;
+            } else {
                 const int base = (j / Ns) * Ns * 16 + (j & (Ns - 1));
 #pragma unroll
-                for (int r = 0; r < 4; ++r) lds[mp_lds<N>(tb, base + (4 * g + r) * Ns)] = y[q][r];
+                for (int r = 0; r < 4; ++r) lds[mp_lds<N>(tb, base + (4 * g + r) * Ns)] = y[r];
             }
-            __syncthreads();
+        }
+        if (!last) {
+            __syncthreads();  // stage outputs complete
 #pragma unroll
             for (int q = 0; q < GPW; ++q) {
                 const int j = (wt * GPW + q) * 16 + c;
@@ -239,6 +242,7 @@ __global__ void __launch_bounds__(256) mfma_rows_kernel(const cpx<float> *in, cp
                     x[q][4 + s] = v.y;
                 }
             }
+            if (S + 2 < NST) __syncthreads();  // reads done before the next stage overwrites
         }
     }
 }
