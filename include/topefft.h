@@ -176,8 +176,9 @@ void topePlanPassTiming(void *plan_impl, int enable);
 int topePlanPassTimes(struct topeFFT *f, void *plan_impl, double *ms, int max_passes);
 /* Mean device time in milliseconds of launch position `pass` alone: HIP
  * events around `reps` back-to-back launches of that pass on f->stream (no
- * events between the launches).  Only in-place passes; returns -1 and sets
- * f->error otherwise. */
+ * events between the launches).  Repeating one pass in isolation leaves the
+ * plan's device data in an undefined state: upload again before the next
+ * transform whose result matters. */
 double topePlanTimePass(struct topeFFT *f, void *plan_impl, int pass, int dir, int reps);
 /* Kernel name of launch position `pass` (static string). */
 const char *topePlanPassName(void *plan_impl, int pass);

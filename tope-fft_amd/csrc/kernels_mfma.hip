@@ -74,9 +74,9 @@ static hipError_t launch_n(const void *in, void *out, const MfmaArgs &a, int sig
     const cpx<float> *i = static_cast<const cpx<float> *>(in);
     cpx<float> *o = static_cast<cpx<float> *>(out);
     if (sign < 0)
-        hipLaunchKernelGGL((mfma_rows_kernel<MODE, N, -1>), dim3((unsigned)blocks), dim3(SH::THREADS), 0, s, i, o, a);
+        hipLaunchKernelGGL((mfma_rows_kernel<MODE, N, -1>), dim3((unsigned)blocks), dim3(256), 0, s, i, o, a);
     else
-        hipLaunchKernelGGL((mfma_rows_kernel<MODE, N, 1>), dim3((unsigned)blocks), dim3(SH::THREADS), 0, s, i, o, a);
+        hipLaunchKernelGGL((mfma_rows_kernel<MODE, N, 1>), dim3((unsigned)blocks), dim3(256), 0, s, i, o, a);
     return hipGetLastError();
 }
 

@@ -138,22 +138,22 @@ __device__ __forceinline__ void dft16_mfma(const float (&x)[8], cpx<float> (&y)[
 template <int N>
 __device__ __forceinline__ int mp_lds(int t, int i) { return t * (N + N / 16) + i + (i >> 4); }
 
-// One block = 8 waves: TPB transforms of length N = 16^NST, WPT waves per
-// transform, GPW groups of 16 butterflies per wave per stage.
+// One block = TPB transforms of length N = 16^NST, WPT waves per transform,
+// GPW groups of 16 butterflies per wave per stage.
 template <int MODE, int N, int SIGN>
 struct MfmaShape {
     static constexpr int NST = ilog2c(N) / 4;
     static constexpr int NB = N / 16;                    // butterflies per stage
     static constexpr int NG = NB / 16;                   // groups per stage
-    static constexpr int WPT = NG >= 8 ? 8 : NG;         // waves per transform
+    static constexpr int WPT = NG >= 4 ? 4 : NG;         // waves per transform
     static constexpr int GPW = NG / WPT;                 // groups per wave
-    static constexpr int TPB = 8 / WPT;                  // transforms per block
-    static constexpr int THREADS = 512;
+    static constexpr int TPB = 4 / WPT;                  // transforms per block (4 waves)
+    static constexpr int THREADS = 256;
     static_assert(N == (1 << (4 * NST)) && NST >= 2, "N must be 16^s, s >= 2");
 };
 
 template <int MODE, int N, int SIGN>
-__global__ void __launch_bounds__(512) mfma_rows_kernel(const cpx<float> *in, cpx<float> *out, const MfmaArgs a)
+__global__ void __launch_bounds__(256) mfma_rows_kernel(const cpx<float> *in, cpx<float> *out, const MfmaArgs a)
 {
     using SH = MfmaShape<MODE, N, SIGN>;
     constexpr int NB = SH::NB, WPT = SH::WPT, GPW = SH::GPW, TPB = SH::TPB, NST = SH::NST;
@@ -187,16 +187,10 @@ __global__ void __launch_bounds__(512) mfma_rows_kernel(const cpx<float> *in, cp
         }
     }
 
-    // Per stage: inputs are in registers (from HBM for stage 0, from LDS
-    // after a barrier otherwise); each group's 4 outputs per lane go to LDS
-    // (or HBM in the last stage) right after its product, so accumulators
-    // and outputs never stay live across groups.
 #pragma unroll
     for (int S = 0; S < NST; ++S) {
         const int Ns = 1 << (4 * S);
-        const bool last = S == NST - 1;
-        __amdgpu_buffer_rsrc_t rout;
-        if (last) rout = make_rsrc(out + row * N, (a.out_elems - row * N) * CB);
+        cpx<float> y[GPW][4];
 #pragma unroll
         for (int q = 0; q < GPW; ++q) {
             const int j = (wt * GPW + q) * 16 + c;
@@ -216,22 +210,29 @@ __global__ void __launch_bounds__(512) mfma_rows_kernel(const cpx<float> *in, cp
                     x[q][4 + s] = v.y;
                 }
             }
-            cpx<float> y[4];
-            dft16_mfma<MODE>(x[q], y, a.k, dir, lane);
-            if (last) {
-                if (valid) {
+            dft16_mfma<MODE>(x[q], y[q], a.k, dir, lane);
+        }
+        if (S == NST - 1) {
+            if (valid) {
+                const __amdgpu_buffer_rsrc_t rout = make_rsrc(out + row * N, (a.out_elems - row * N) * CB);
+#pragma unroll
+                for (int q = 0; q < GPW; ++q) {
+                    const int j = (wt * GPW + q) * 16 + c;
 #pragma unroll
                     for (int r = 0; r < 4; ++r)
-                        buf_store<float>(scale(y[r], a.scale), rout, (j + NB * (4 * g + r)) * CB, 0);
+                        buf_store<float>(scale(y[q][r], a.scale), rout, (j + NB * (4 * g + r)) * CB, 0);
                 }
-            } else {
+            }
+        } else {
+            if (S > 0) __syncthreads();
+#pragma unroll
+            for (int q = 0; q < GPW; ++q) {
+                const int j = (wt * GPW + q) * 16 + c;
                 const int base = (j / Ns) * Ns * 16 + (j & (Ns - 1));
 #pragma unroll
-                for (int r = 0; r < 4; ++r) lds[mp_lds<N>(tb, base + (4 * g + r) * Ns)] = y[r];
+                for (int r = 0; r < 4; ++r) lds[mp_lds<N>(tb, base + (4 * g + r) * Ns)] = y[q][r];
             }
-        }
-        if (!last) {
-            __syncthreads();  // stage outputs complete
+            __syncthreads();
 #pragma unroll
             for (int q = 0; q < GPW; ++q) {
                 const int j = (wt * GPW + q) * 16 + c;
@@ -242,7 +243,6 @@ __global__ void __launch_bounds__(512) mfma_rows_kernel(const cpx<float> *in, cp
                     x[q][4 + s] = v.y;
                 }
             }
-            if (S + 2 < NST) __syncthreads();  // reads done before the next stage overwrites
         }
     }
 }
