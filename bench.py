@@ -115,6 +115,44 @@ def mixed_precision_leg(fw, steps, warmup):
     return res
 
 
+def uneven_batch_leg(fw, steps, warmup):
+    """20,000 transforms, lengths 2^6..2^14 log-uniform, packed with an
+    offset table: one persistent launch (direct and hipGraph replay, after
+    autotuning) against one launch per transform (direct and graph)."""
+    import topefft
+    import workloads
+    sizes = workloads.uneven_sizes(workloads.SEED_MANY_SIZES, 20000)
+    total = int(sizes.sum())
+    x = workloads.complex_uniform(workloads.SEED_MANY_DATA, total)
+    res = {"workload": "20000 x 1D C2C, n = 2^6..2^14 log-uniform (seeded), packed, forward",
+           "elements": total}
+    for prec, name, xs, sb in ((topefft.PREC_F64, "fp64", x, 16),
+                               (topefft.PREC_F32, "complex64", x.astype(np.complex64), 8)):
+        p = topefft.ManyPlan(fw, sizes, xs, prec)
+        p.tune(None, reps=3)
+        radix, grid = p.schedule()
+        entry = {"schedule_radix_by_log2n": {str(L): radix[L] for L in range(6, 15)}, "grid": grid}
+        for m in (topefft.MANY_PERSISTENT, topefft.MANY_PERSISTENT_GRAPH, topefft.MANY_PER_TRANSFORM,
+                  topefft.MANY_PER_TRANSFORM_GRAPH):
+            p.set_method(m)
+            k = steps if m in (topefft.MANY_PERSISTENT, topefft.MANY_PERSISTENT_GRAPH) else 3
+            dirs = [topefft.FORWARD, topefft.INVERSE]
+            for i in range(max(1, warmup)):
+                p.exec_device(dirs[i % 2])
+            fw.synchronize()
+            t0 = time.perf_counter()
+            for i in range(k):
+                p.exec_device(dirs[i % 2])
+            fw.synchronize()
+            t = (time.perf_counter() - t0) / k
+            entry[topefft.MANY_METHOD_NAMES[m]] = {"ms": round(t * 1e3, 4),
+                                                    "gbs": round(gbs(2 * sb * total, t), 1),
+                                                    "transforms_per_s": round(len(sizes) / t, 1)}
+        p.destroy()
+        res[name] = entry
+    return res
+
+
 def cpu_baseline(x, dims, budget_s=12.0):
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle_py
@@ -172,6 +210,7 @@ def main():
     del x3
 
     mp = mixed_precision_leg(fw, max(5, args.steps // 2), args.warmup)
+    many = uneven_batch_leg(fw, max(5, args.steps // 2), args.warmup)
 
     cpu = None
     if not args.no_cpu:
@@ -204,7 +243,8 @@ def main():
                    "3d_256cubed": {"gbs": round(gbs(alg3, t3), 2), "ms": round(t3 * 1e3, 4),
                                    "transforms_per_s": round(1.0 / t3, 2),
                                    "pass_ms": [round(float(m), 4) for m in pass3]},
-                   "mixed_precision_batch": mp},
+                   "mixed_precision_batch": mp,
+                   "uneven_batch": many},
         "roofline": {"bound": "hbm", "kernel": names2[dom], "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": profiled_traffic(names2[dom]),

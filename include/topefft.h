@@ -221,9 +221,52 @@ void topeBatchExec(struct topeFFT *f, struct topePlanBatch *t, void *host, int d
 void topeBatchExecDevice(struct topeFFT *f, struct topePlanBatch *t, int dir);
 void topeBatchDestroy(struct topeFFT *f, struct topePlanBatch *t);
 
+/* ---- packed batch of uneven power-of-two lengths (no reference
+ * counterpart) ---------------------------------------------------------
+ * `count` transforms with lengths sizes[i] (powers of two, 16 .. 16384)
+ * stored back to back: transform i starts at element sum_{k<i} sizes[k]
+ * (topeManyOffsets).  Methods:
+ *   TOPE_MANY_PERSISTENT            one persistent kernel launch for all
+ *   TOPE_MANY_PERSISTENT_GRAPH      that launch replayed from a hipGraph
+ *   TOPE_MANY_PER_TRANSFORM         one kernel launch per transform
+ *   TOPE_MANY_PER_TRANSFORM_GRAPH   those launches replayed from one hipGraph
+ * topeManyTune times the candidate radix schedules of every length class
+ * (radix-16 or radix-8 stages) and the resident blocks per CU on the device,
+ * keeps the fastest, and appends them to the wisdom file `wisdom_path`;
+ * TOPEFFT_WISDOM=<path> makes PlanInit reuse them.  Tuning transforms the
+ * device data repeatedly and restores it afterwards. */
+#define TOPE_MANY_PERSISTENT 0
+#define TOPE_MANY_PERSISTENT_GRAPH 1
+#define TOPE_MANY_PER_TRANSFORM 2
+#define TOPE_MANY_PER_TRANSFORM_GRAPH 3
+
+struct topePlanMany {
+    int count;
+    int precision; /* TOPE_PREC_F64 (complex128) or TOPE_PREC_F32 (complex64) */
+    int method;
+    int reserved;
+    void *data; /* device buffer, packed transforms */
+    size_t dataSize;
+    long long elements;
+    tope_ulong totalMemory;
+    tope_ulong totalPreKernel;
+    tope_ulong totalKernel;
+    void *impl;
+};
+
+void topeManyPlanInit(struct topeFFT *f, struct topePlanMany *t, int count, const int *sizes, int precision,
+                      const void *host);
+void topeManySetMethod(struct topeFFT *f, struct topePlanMany *t, int method);
+void topeManyExec(struct topeFFT *f, struct topePlanMany *t, void *host, int dir);
+void topeManyExecDevice(struct topeFFT *f, struct topePlanMany *t, int dir);
+long long topeManyOffsets(struct topePlanMany *t, long long *offsets);
+void topeManyGetSchedule(struct topePlanMany *t, int *radix_by_log2, int *grid);
+int topeManyTune(struct topeFFT *f, struct topePlanMany *t, const char *wisdom_path, int reps);
+void topeManyDestroy(struct topeFFT *f, struct topePlanMany *t);
+
 /* Size of the public structs, so foreign-language bindings can check their
  * mirrors of this header. which: 0 topeFFT, 1 topePlan1D, 2 topePlan2D,
- * 3 topePlan3D, 4 topePlanBatch. */
+ * 3 topePlan3D, 4 topePlanBatch, 5 topePlanMany. */
 size_t topeStructSize(int which);
 
 /* Human-readable message for a status code. */

@@ -752,6 +752,26 @@ void destroy_common(struct topeFFT *f, PlanImpl *P)
 
 }  // namespace
 
+// Framework plumbing shared with the uneven-batch module (many.cpp).
+namespace tope {
+hipStream_t fw_stream(struct topeFFT *f)
+{
+    FrameworkImpl *fw = fw_of(f);
+    return fw ? fw->stream : nullptr;
+}
+void *fw_handle(struct topeFFT *f) { return fw_of(f); }
+void fw_retain(struct topeFFT *f)
+{
+    if (FrameworkImpl *fw = fw_of(f)) fw->live_plans++;
+}
+void fw_release(struct topeFFT *f, void *owner)
+{
+    FrameworkImpl *fw = static_cast<FrameworkImpl *>(owner);
+    if (fw && fw_of(f) == fw && --fw->live_plans <= 0) release_framework(f);
+}
+const AxisVariant *row_variant(int prec, int n) { return select_variant(prec, n, false, 1); }
+}  // namespace tope
+
 extern "C" {
 
 void topeFFTInit(struct topeFFT *f)
@@ -1150,6 +1170,7 @@ size_t topeStructSize(int which)
     case 2: return sizeof(struct topePlan2D);
     case 3: return sizeof(struct topePlan3D);
     case 4: return sizeof(struct topePlanBatch);
+    case 5: return sizeof(struct topePlanMany);
     default: return 0;
     }
 }

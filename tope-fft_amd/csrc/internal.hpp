@@ -4,6 +4,8 @@
 
 #include "axis_kernel.hpp"
 
+struct topeFFT;
+
 namespace tope {
 
 enum { PREC_F64 = 0, PREC_F32 = 1 };
@@ -62,5 +64,16 @@ hipError_t gen_mfma_constants(void *out, hipStream_t s);
 size_t mfma_constants_bytes();
 bool mfma_supported(int n);
 hipError_t launch_mfma_rows(int mode, int n, const void *in, void *out, const MfmaArgs &a, int sign, hipStream_t s);
+
+// Framework plumbing (api.cpp) for modules outside it.
+hipStream_t fw_stream(struct topeFFT *f);
+void *fw_handle(struct topeFFT *f);
+void fw_retain(struct topeFFT *f);
+void fw_release(struct topeFFT *f, void *owner);
+// Row variant the planner would pick for one transform of length n.
+const AxisVariant *row_variant(int prec, int n);
+
+// Persistent uneven batch (many_kernel.hpp); args is ManyArgs<double|float>.
+hipError_t launch_many(int prec, const void *in, void *out, const void *args, int grid, int sign, hipStream_t s);
 
 }  // namespace tope

@@ -1,0 +1,127 @@
+// Persistent kernel for a packed batch of transforms of uneven power-of-two
+// lengths (no reference counterpart: the reference plans one length per
+// topeFFTInit, src/fft1d.c:303-370).
+//
+// Transforms are grouped by length class (n = 2^L, L = 4..14).  A task is up
+// to TB transforms of one class, TB = 1024 / P threads' worth (P = n / E
+// threads per transform), so every task moves the same ~16384 elements; the
+// task list is ordered by class (largest first) and dealt round-robin to
+// the resident blocks, which loop until the list is exhausted (every block
+// reaches the end; no inter-block waits).  Each class runs a compile-time radix
+// schedule (E = 16: radix-16 stages; E = 8: radix-8 stages) chosen per class
+// at plan time, by the autotuner or by default.
+#pragma once
+#include "axis_kernel.hpp"
+
+namespace tope {
+
+constexpr int MANY_THREADS = 1024;
+constexpr int MANY_LMIN = 4, MANY_LMAX = 14;
+
+struct ManyTask {
+    int cls;    // log2 n
+    int first;  // index into order[]
+    int count;  // transforms in this task (<= TB of the class schedule)
+    int pad;
+};
+
+template <typename R>
+struct ManyArgs {
+    const ManyTask *tasks;
+    int ntasks;
+    int seg0[16], seg1[16];  // tasks of class L are tasks[seg0[L] .. seg1[L])
+    const int *order;       // transform ids grouped by class
+    const long long *offs;  // element offset of each transform in the packed buffer
+    const cpx<R> *tw[16];   // stage twiddles of each class (layout of its schedule)
+    int sched[16];          // per class: 0 -> E = 16, 1 -> E = 8
+    long long elems;        // packed buffer length (elements)
+};
+
+template <typename R, int N, int E, int SIGN>
+__device__ __forceinline__ void many_rows(const cpx<R> *in, cpx<R> *out, const ManyArgs<R> &a, const ManyTask &tk,
+                                          void *lds)
+{
+    constexpr int P = N / E;
+    constexpr int TB = MANY_THREADS / P;
+    using SC = Sched<N, E>;
+    constexpr int CB = (int)sizeof(cpx<R>);
+    const int tid = threadIdx.x;
+    const int t = tid / P, p = tid % P;
+    const bool valid = t < tk.count;
+    const long long off = valid ? a.offs[a.order[tk.first + t]] : 0;
+    const __amdgpu_buffer_rsrc_t rsrc = make_rsrc(in, a.elems * CB);
+    const __amdgpu_buffer_rsrc_t wsrc = make_rsrc(out, a.elems * CB);
+    const int voff = (int)((off + p) * CB);
+    constexpr int sstep = P * CB;
+    cpx<R> v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) v[e] = valid ? buf_load<R>(rsrc, voff, e * sstep) : cpx<R>{0, 0};
+    run_stages<R, N, E, TB, false, true, SIGN, 0>(v, p, t, a.tw[SC::LN], lds);
+    const R sc = SIGN > 0 ? (R)(1.0 / N) : (R)1;  // INVERSE: 1/n, a power of two (exact)
+    if (valid) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) buf_store<R>(scale(v[e], sc), wsrc, voff, e * sstep);
+    }
+}
+
+// LDS: TB * (N + N/16) slots of one component for every class = 17408 words.
+template <typename R>
+constexpr int many_lds_bytes() { return MANY_THREADS / 16 * 17 * 16 * (int)sizeof(R); }
+
+// Dispatch one task of class L to its schedule (E = 8 only where n/8 threads
+// fit the block).
+template <typename R, int L, int SIGN>
+__device__ __forceinline__ void many_class(const cpx<R> *in, cpx<R> *out, const ManyArgs<R> &a, const ManyTask &tk,
+                                           void *lds, bool e8)
+{
+    constexpr int N = 1 << L;
+    if constexpr (N >= 8 && N / 8 <= MANY_THREADS) {
+        if (e8) {
+            many_rows<R, N, 8, SIGN>(in, out, a, tk, lds);
+            return;
+        }
+    }
+    many_rows<R, N, (N < 16 ? N : 16), SIGN>(in, out, a, tk, lds);
+}
+
+// Tasks of one class: this block takes global task indices b, b + G, ...
+// that fall in [seg0[L], seg1[L]) -- the same round-robin deal as one loop
+// over the whole list, but each loop body holds one class's code, which keeps
+// the register allocation of the 1024-thread block free of spills.
+template <typename R, int L, int SIGN>
+__device__ __forceinline__ void many_segment(const cpx<R> *in, cpx<R> *out, const ManyArgs<R> &a, void *lds)
+{
+    const int s0 = a.seg0[L], s1 = a.seg1[L];
+    if (s0 >= s1) return;
+    const int G = gridDim.x, b = blockIdx.x;
+    int task = s0 + ((b - s0) % G + G) % G;
+    const bool e8 = a.sched[L] != 0;
+    for (; task < s1; task += G) {
+        ManyTask tk = a.tasks[task];
+        // the task is block-uniform: keep its fields in scalar registers
+        tk.cls = L;
+        tk.first = __builtin_amdgcn_readfirstlane(tk.first);
+        tk.count = __builtin_amdgcn_readfirstlane(tk.count);
+        many_class<R, L, SIGN>(in, out, a, tk, lds, e8);
+        __syncthreads();  // the next task reuses the LDS
+    }
+}
+
+template <typename R, int SIGN>
+__global__ void __launch_bounds__(MANY_THREADS) many_kernel(const cpx<R> *in, cpx<R> *out, const ManyArgs<R> a)
+{
+    __shared__ __align__(16) unsigned char lds[many_lds_bytes<R>()];
+    many_segment<R, 14, SIGN>(in, out, a, lds);
+    many_segment<R, 13, SIGN>(in, out, a, lds);
+    many_segment<R, 12, SIGN>(in, out, a, lds);
+    many_segment<R, 11, SIGN>(in, out, a, lds);
+    many_segment<R, 10, SIGN>(in, out, a, lds);
+    many_segment<R, 9, SIGN>(in, out, a, lds);
+    many_segment<R, 8, SIGN>(in, out, a, lds);
+    many_segment<R, 7, SIGN>(in, out, a, lds);
+    many_segment<R, 6, SIGN>(in, out, a, lds);
+    many_segment<R, 5, SIGN>(in, out, a, lds);
+    many_segment<R, 4, SIGN>(in, out, a, lds);
+}
+
+}  // namespace tope

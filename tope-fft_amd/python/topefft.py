@@ -60,6 +60,16 @@ class TopePlanBatch(C.Structure):
                 ("totalPreKernel", C.c_uint64), ("totalKernel", C.c_uint64), ("impl", C.c_void_p)]
 
 
+class TopePlanMany(C.Structure):
+    _fields_ = [("count", C.c_int), ("precision", C.c_int), ("method", C.c_int), ("reserved", C.c_int),
+                ("data", C.c_void_p), ("dataSize", C.c_size_t), ("elements", C.c_longlong),
+                ("totalMemory", C.c_uint64), ("totalPreKernel", C.c_uint64), ("totalKernel", C.c_uint64),
+                ("impl", C.c_void_p)]
+
+
+MANY_PERSISTENT, MANY_PERSISTENT_GRAPH, MANY_PER_TRANSFORM, MANY_PER_TRANSFORM_GRAPH = 0, 1, 2, 3
+MANY_METHOD_NAMES = {0: "persistent", 1: "persistent_graph", 2: "per_transform", 3: "per_transform_graph"}
+
 PREC_F64, PREC_F32 = 0, 1
 MODE_VALU, MODE_MFMA_BF16X3, MODE_MFMA_BF16X6, MODE_MFMA_FP32 = 0, 1, 2, 3
 MODE_NAMES = {MODE_VALU: "valu", MODE_MFMA_BF16X3: "mfma_bf16x3", MODE_MFMA_BF16X6: "mfma_bf16x6",
@@ -69,6 +79,7 @@ MODE_NAMES = {MODE_VALU: "valu", MODE_MFMA_BF16X3: "mfma_bf16x3", MODE_MFMA_BF16
 _F, _P1, _P2, _P3 = C.POINTER(TopeFFT), C.POINTER(TopePlan1D), C.POINTER(TopePlan2D), C.POINTER(TopePlan3D)
 _DP = C.POINTER(C.c_double)
 _PB = C.POINTER(TopePlanBatch)
+_PM = C.POINTER(TopePlanMany)
 SIGNATURES = {
     "topeFFTInit": (None, [_F]),
     "topeFFTDestroy": (None, [_F]),
@@ -98,6 +109,14 @@ SIGNATURES = {
     "topeBatchExec": (None, [_F, _PB, C.c_void_p, C.c_int]),
     "topeBatchExecDevice": (None, [_F, _PB, C.c_int]),
     "topeBatchDestroy": (None, [_F, _PB]),
+    "topeManyPlanInit": (None, [_F, _PM, C.c_int, C.POINTER(C.c_int), C.c_int, C.c_void_p]),
+    "topeManySetMethod": (None, [_F, _PM, C.c_int]),
+    "topeManyExec": (None, [_F, _PM, C.c_void_p, C.c_int]),
+    "topeManyExecDevice": (None, [_F, _PM, C.c_int]),
+    "topeManyOffsets": (C.c_longlong, [_PM, C.POINTER(C.c_longlong)]),
+    "topeManyGetSchedule": (None, [_PM, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "topeManyTune": (C.c_int, [_F, _PM, C.c_char_p, C.c_int]),
+    "topeManyDestroy": (None, [_F, _PM]),
     "topeErrorString": (C.c_char_p, [C.c_int]),
     "topePlanTwiddles": (C.c_longlong, [_F, C.c_void_p, C.c_int, _DP, C.c_longlong]),
 }
@@ -123,8 +142,8 @@ def load(path=LIB_PATH):
 def check_layout():
     lib = load()
     mine = [C.sizeof(TopeFFT), C.sizeof(TopePlan1D), C.sizeof(TopePlan2D), C.sizeof(TopePlan3D),
-            C.sizeof(TopePlanBatch)]
-    theirs = [lib.topeStructSize(i) for i in range(5)]
+            C.sizeof(TopePlanBatch), C.sizeof(TopePlanMany)]
+    theirs = [lib.topeStructSize(i) for i in range(6)]
     return mine, theirs
 
 
@@ -303,6 +322,59 @@ class BatchPlan:
     def destroy(self):
         if self.t.impl:
             self.lib.topeBatchDestroy(C.byref(self.fw.f), C.byref(self.t))
+
+
+class ManyPlan:
+    """Packed batch of uneven power-of-two lengths (topeManyPlanInit)."""
+
+    def __init__(self, fw, sizes, x, precision=PREC_F64):
+        self.fw, self.lib = fw, fw.lib
+        self.sizes = np.ascontiguousarray(sizes, dtype=np.int32)
+        self.dtype = np.complex128 if precision == PREC_F64 else np.complex64
+        host = np.ascontiguousarray(x, dtype=self.dtype).reshape(-1)
+        if host.size != int(self.sizes.sum()):
+            raise ValueError("packed input length does not match sum(sizes)")
+        self.host = host
+        self.t = TopePlanMany()
+        self.lib.topeManyPlanInit(C.byref(fw.f), C.byref(self.t), len(self.sizes),
+                                  self.sizes.ctypes.data_as(C.POINTER(C.c_int)), precision,
+                                  host.ctypes.data_as(C.c_void_p))
+        fw.check(f"topeManyPlanInit(count={len(self.sizes)})")
+
+    def offsets(self):
+        out = np.empty(len(self.sizes), dtype=np.int64)
+        self.lib.topeManyOffsets(C.byref(self.t), out.ctypes.data_as(C.POINTER(C.c_longlong)))
+        return out
+
+    def set_method(self, m):
+        self.lib.topeManySetMethod(C.byref(self.fw.f), C.byref(self.t), m)
+        self.fw.check("topeManySetMethod")
+
+    def exec(self, direction=FORWARD):
+        out = np.empty(self.host.size, dtype=self.dtype)
+        self.lib.topeManyExec(C.byref(self.fw.f), C.byref(self.t), out.ctypes.data_as(C.c_void_p), direction)
+        self.fw.check("topeManyExec")
+        return out
+
+    def exec_device(self, direction=FORWARD):
+        self.lib.topeManyExecDevice(C.byref(self.fw.f), C.byref(self.t), direction)
+        self.fw.check("topeManyExecDevice")
+
+    def schedule(self):
+        r = (C.c_int * 16)()
+        g = C.c_int()
+        self.lib.topeManyGetSchedule(C.byref(self.t), r, C.byref(g))
+        return list(r), g.value
+
+    def tune(self, wisdom_path=None, reps=5):
+        rc = self.lib.topeManyTune(C.byref(self.fw.f), C.byref(self.t),
+                                   wisdom_path.encode() if wisdom_path else None, reps)
+        self.fw.check("topeManyTune")
+        return rc
+
+    def destroy(self):
+        if self.t.impl:
+            self.lib.topeManyDestroy(C.byref(self.fw.f), C.byref(self.t))
 
 
 def transform(x, dims, direction=FORWARD):

@@ -69,6 +69,12 @@ def complex_gaussian(seed, n):
     return out
 
 
+def uneven_sizes(seed, count, lmin=6, lmax=14):
+    """`count` powers of two drawn log-uniform from 2^lmin .. 2^lmax."""
+    z = splitmix64(seed, count)
+    return (1 << (lmin + (z % np.uint64(lmax - lmin + 1)).astype(np.int64))).astype(np.int32)
+
+
 def planted_peaks(seed, dims, count=8):
     """`count` distinct frequency triples (kx, ky, kz) and phases drawn from
     the seed's second stream; dims = (nx, ny, nz)."""
