@@ -153,6 +153,44 @@ def uneven_batch_leg(fw, steps, warmup):
     return res
 
 
+def large_3d_leg(fw, steps, warmup):
+    """3D 512^3: Gaussian random field with |k|^-2 power spectrum (k-space
+    synthesis, inverse transform by the FP64 plan) plus 1e-3 white noise;
+    FP64 (2 GiB) and complex64 (1 GiB) plans, forward then inverse."""
+    import topefft
+    import workloads
+    dims = (512, 512, 512)
+    n = 512 ** 3
+    spec = workloads.powerlaw_spectrum(workloads.SEED_3D_LARGE, dims)
+    p = topefft.Plan(fw, dims, spec)
+    field = p.exec(topefft.INVERSE)
+    p.destroy()
+    del spec
+    field += 1e-3 * workloads.complex_gaussian(workloads.SEED_3D_LARGE + 1, n)
+    res = {"workload": "3D C2C 512^3, |k|^-2 Gaussian random field + 1e-3 white noise, forward then inverse"}
+    ref = None
+    for prec, name, sb in ((None, "fp64", 16), (topefft.PREC_F32, "complex64", 8)):
+        xs = field if prec is None else field.astype(np.complex64)
+        q = topefft.Plan(fw, dims, xs, precision=prec)
+        X = q.exec(topefft.FORWARD)
+        back = q.exec(topefft.INVERSE)
+        t, pass_ms = measure(fw, q, steps, warmup)
+        names = q.pass_names()
+        q.destroy()
+        Xd = X.astype(np.complex128)
+        entry = {"ms_per_transform": round(t * 1e3, 4), "gbs": round(gbs(2 * sb * n * 3, t), 1),
+                 "passes": [{"kernel": nm, "ms": round(float(m), 4), "gbs": round(gbs(2 * sb * n, m * 1e-3), 1)}
+                            for nm, m in zip(names, pass_ms)],
+                 "round_trip_rel_l2": float(f"{np.linalg.norm(back.astype(np.complex128) - field) / np.linalg.norm(field):.3e}")}
+        if ref is None:
+            ref = Xd
+        else:
+            entry["rel_l2_vs_fp64"] = float(f"{np.linalg.norm(Xd - ref) / np.linalg.norm(ref):.3e}")
+        del X, back, Xd, xs
+        res[name] = entry
+    return res
+
+
 def cpu_baseline(x, dims, budget_s=12.0):
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle_py
@@ -211,6 +249,7 @@ def main():
 
     mp = mixed_precision_leg(fw, max(5, args.steps // 2), args.warmup)
     many = uneven_batch_leg(fw, max(5, args.steps // 2), args.warmup)
+    big3 = large_3d_leg(fw, 4, 1)
 
     cpu = None
     if not args.no_cpu:
@@ -244,7 +283,8 @@ def main():
                                    "transforms_per_s": round(1.0 / t3, 2),
                                    "pass_ms": [round(float(m), 4) for m in pass3]},
                    "mixed_precision_batch": mp,
-                   "uneven_batch": many},
+                   "uneven_batch": many,
+                   "3d_512cubed": big3},
         "roofline": {"bound": "hbm", "kernel": names2[dom], "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": profiled_traffic(names2[dom]),

@@ -221,6 +221,18 @@ void topeBatchExec(struct topeFFT *f, struct topePlanBatch *t, void *host, int d
 void topeBatchExecDevice(struct topeFFT *f, struct topePlanBatch *t, int dir);
 void topeBatchDestroy(struct topeFFT *f, struct topePlanBatch *t);
 
+/* 2D/3D plans in a chosen precision: TOPE_PREC_F64 (complex128, host
+ * double*, identical to tope2DPlanInit/tope3DPlanInit) or TOPE_PREC_F32
+ * (complex64 storage, host float*, FP32 butterflies with FP64-generated
+ * twiddles).  Exec/ExecDevice/Destroy as for the reference plans; the Ex
+ * Exec variants take the host buffer as void*. */
+void tope2DPlanInitEx(struct topeFFT *f, struct topePlan2D *t, int x, int y, int type, int precision,
+                      const void *host);
+void tope3DPlanInitEx(struct topeFFT *f, struct topePlan3D *t, int x, int y, int z, int type, int precision,
+                      const void *host);
+void tope2DExecEx(struct topeFFT *f, struct topePlan2D *t, void *host, int dir);
+void tope3DExecEx(struct topeFFT *f, struct topePlan3D *t, void *host, int dir);
+
 /* ---- packed batch of uneven power-of-two lengths (no reference
  * counterpart) ---------------------------------------------------------
  * `count` transforms with lengths sizes[i] (powers of two, 16 .. 16384)

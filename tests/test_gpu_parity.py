@@ -267,3 +267,41 @@ def test_2d_dif2_schedule_small(oracle, monkeypatch, dims):
     p.destroy()
     assert rel_l2(X, oracle.dft(x, dims, 1)) < TOL
     assert rel_l2(back, x) < TOL
+
+
+@pytest.mark.parametrize("dims", [(16, 16), (64, 32), (8, 16, 4), (32, 32, 32), (64, 4, 8)],
+                         ids=lambda d: "x".join(map(str, d)))
+def test_nd_complex64(oracle, dims):
+    # complex64 storage plans (tope2DPlanInitEx / tope3DPlanInitEx)
+    x = seeded(dims, salt=9).astype(np.complex64)
+    x64 = x.astype(np.complex128)
+    fw = topefft.Framework()
+    p = topefft.Plan(fw, dims, x, precision=topefft.PREC_F32)
+    X = p.exec(topefft.FORWARD).astype(np.complex128)
+    back = p.exec(topefft.INVERSE).astype(np.complex128)
+    p.destroy()
+    assert rel_l2(X, oracle.dft(x64, dims, 1)) < 1e-6
+    assert rel_l2(back, x64) < 1e-6
+
+
+def test_3d_512_fp64_and_complex64():
+    # the 512^3 configuration: FP64 forward then inverse round trip, and the
+    # complex64 plan against the FP64 forward result
+    dims = (512, 512, 512)
+    n = 512 ** 3
+    x = workloads.complex_uniform(workloads.SEED_3D_LARGE, n)
+    fw = topefft.Framework()
+    p = topefft.Plan(fw, dims, x)
+    X = p.exec(topefft.FORWARD)
+    back = p.exec(topefft.INVERSE)
+    p.destroy()
+    assert rel_l2(back, x) < TOL
+    # Parseval on the FP64 spectrum
+    assert abs(np.vdot(X, X).real - n * np.vdot(x, x).real) / (n * np.vdot(x, x).real) < TOL
+    fw = topefft.Framework()
+    q = topefft.Plan(fw, dims, x.astype(np.complex64), precision=topefft.PREC_F32)
+    X32 = q.exec(topefft.FORWARD)
+    q.destroy()
+    err = rel_l2(X32.astype(np.complex128), X)
+    print(f"512^3 complex64 vs FP64 rel-L2 {err:.3e}")
+    assert err < 1e-6

@@ -568,6 +568,7 @@ int build_plan(struct topeFFT *f, int shape, long long nx, long long ny, long lo
     if (shape == SHAPE_BATCH && !is_pow2(nx)) return TOPE_ERROR_INVALID_SIZE;
     if (prec != PREC_F64 && prec != PREC_F32) return TOPE_ERROR_INVALID_ARG;
     if (mode != MP_VALU && (prec != PREC_F32 || shape != SHAPE_BATCH)) return TOPE_ERROR_UNSUPPORTED;
+    if (prec == PREC_F32 && shape == 1 && !is_pow2(nx)) return TOPE_ERROR_UNSUPPORTED;
     if (mode != MP_VALU && mode != MP_BF16X3 && mode != MP_BF16X6 && mode != MP_FP32) return TOPE_ERROR_INVALID_ARG;
     if (mode != MP_VALU && !mfma_supported((int)nx)) return TOPE_ERROR_UNSUPPORTED;
     if (!d) return TOPE_ERROR_INVALID_ARG;
@@ -993,6 +994,89 @@ void tope3DDestroy(struct topeFFT *f, struct topePlan3D *t)
     t->impl = nullptr;
     t->data = t->scratch = t->twdX = t->twdY = t->twdZ = nullptr;
     f->error = TOPE_SUCCESS;
+}
+
+/* ---- 2D/3D plans in a chosen precision ------------------------------------ */
+
+void tope2DPlanInitEx(struct topeFFT *f, struct topePlan2D *t, int x, int y, int type, int precision,
+                      const void *host)
+{
+    if (!f || !t) return;
+    std::memset(t, 0, sizeof(*t));
+    t->x = x;
+    t->y = y;
+    t->type = type;
+    t->dim = 2;
+    ref_radix(x, &t->radX, &t->logX);
+    ref_radix(y, &t->radY, &t->logY);
+    PlanImpl *P = nullptr;
+    int rc = build_plan(f, 2, x, y, 1, precision == TOPE_PREC_F32 ? PREC_F32 : PREC_F64, MP_VALU, host, &P,
+                        &t->totalPreKernel, &t->totalMemory);
+    if (!rc && precision != TOPE_PREC_F64 && precision != TOPE_PREC_F32) rc = TOPE_ERROR_INVALID_ARG;
+    f->error = rc;
+    if (rc) {
+        if (P) free_plan(P);
+        return;
+    }
+    P->fw->live_plans++;
+    t->impl = P;
+    t->dataSize = P->bytes;
+    t->twdX = P->axis_tw[0].table;
+    t->twdY = P->axis_tw[1].table;
+    sync_fields(t, P);
+}
+
+void tope3DPlanInitEx(struct topeFFT *f, struct topePlan3D *t, int x, int y, int z, int type, int precision,
+                      const void *host)
+{
+    if (!f || !t) return;
+    std::memset(t, 0, sizeof(*t));
+    t->x = x;
+    t->y = y;
+    t->z = z;
+    t->type = type;
+    t->dim = 3;
+    ref_radix(x, &t->radX, &t->logX);
+    ref_radix(y, &t->radY, &t->logY);
+    ref_radix(z, &t->radZ, &t->logZ);
+    PlanImpl *P = nullptr;
+    int rc = build_plan(f, 3, x, y, z, precision == TOPE_PREC_F32 ? PREC_F32 : PREC_F64, MP_VALU, host, &P,
+                        &t->totalPreKernel, &t->totalMemory);
+    if (!rc && precision != TOPE_PREC_F64 && precision != TOPE_PREC_F32) rc = TOPE_ERROR_INVALID_ARG;
+    f->error = rc;
+    if (rc) {
+        if (P) free_plan(P);
+        return;
+    }
+    P->fw->live_plans++;
+    t->impl = P;
+    t->dataSize = P->bytes;
+    t->twdX = P->axis_tw[0].table;
+    t->twdY = P->axis_tw[1].table;
+    t->twdZ = P->axis_tw[2].table;
+    sync_fields(t, P);
+}
+
+void tope2DExecEx(struct topeFFT *f, struct topePlan2D *t, void *host, int dir)
+{
+    if (!f || !t || !t->impl) {
+        if (f) f->error = TOPE_ERROR_INVALID_ARG;
+        return;
+    }
+    PlanImpl *P = impl_of(t->impl);
+    f->error = exec_host(f, P, host, dir, &t->totalKernel, &t->totalMemory);
+    sync_fields(t, P);
+}
+
+void tope3DExecEx(struct topeFFT *f, struct topePlan3D *t, void *host, int dir)
+{
+    if (!f || !t || !t->impl) {
+        if (f) f->error = TOPE_ERROR_INVALID_ARG;
+        return;
+    }
+    PlanImpl *P = impl_of(t->impl);
+    f->error = exec_host(f, P, host, dir, &t->totalKernel, &t->totalMemory);
+    sync_fields(t, P);
 }
 
 /* ---- batched 1D transforms ---------------------------------------------- */

@@ -109,6 +109,10 @@ SIGNATURES = {
     "topeBatchExec": (None, [_F, _PB, C.c_void_p, C.c_int]),
     "topeBatchExecDevice": (None, [_F, _PB, C.c_int]),
     "topeBatchDestroy": (None, [_F, _PB]),
+    "tope2DPlanInitEx": (None, [_F, _P2, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]),
+    "tope3DPlanInitEx": (None, [_F, _P3, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]),
+    "tope2DExecEx": (None, [_F, _P2, C.c_void_p, C.c_int]),
+    "tope3DExecEx": (None, [_F, _P3, C.c_void_p, C.c_int]),
     "topeManyPlanInit": (None, [_F, _PM, C.c_int, C.POINTER(C.c_int), C.c_int, C.c_void_p]),
     "topeManySetMethod": (None, [_F, _PM, C.c_int]),
     "topeManyExec": (None, [_F, _PM, C.c_void_p, C.c_int]),
@@ -187,16 +191,31 @@ class Plan:
     or flat; PlanInit uploads it, exec() transforms and downloads like the
     reference, exec_device() transforms on the device only."""
 
-    def __init__(self, fw, dims, x, type_=C2C):
+    def __init__(self, fw, dims, x, type_=C2C, precision=None):
         self.fw, self.lib = fw, fw.lib
         self.dims = tuple(int(d) for d in dims)
         self.ndim = len(self.dims)
         n = int(np.prod(self.dims))
-        host = np.ascontiguousarray(x, dtype=np.complex128).reshape(-1)
+        self.precision = precision
+        self.dtype = np.complex64 if precision == PREC_F32 else np.complex128
+        host = np.ascontiguousarray(x, dtype=self.dtype).reshape(-1)
         if host.size != n:
             raise ValueError("input size does not match dims")
         self.n = n
-        self.host = host.view(np.float64)
+        self.host = host.view(np.float32 if precision == PREC_F32 else np.float64)
+        if precision is not None:
+            if self.ndim == 2:
+                self.t = TopePlan2D()
+                self.lib.tope2DPlanInitEx(C.byref(fw.f), C.byref(self.t), self.dims[0], self.dims[1], type_,
+                                          precision, host.ctypes.data_as(C.c_void_p))
+            elif self.ndim == 3:
+                self.t = TopePlan3D()
+                self.lib.tope3DPlanInitEx(C.byref(fw.f), C.byref(self.t), self.dims[0], self.dims[1],
+                                          self.dims[2], type_, precision, host.ctypes.data_as(C.c_void_p))
+            else:
+                raise ValueError("precision plans are 2D or 3D (1D: BatchPlan)")
+            fw.check(f"tope{self.ndim}DPlanInitEx{self.dims}")
+            return
         if self.ndim == 1:
             self.t = TopePlan1D()
             self.lib.tope1DPlanInit(C.byref(fw.f), C.byref(self.t), self.dims[0], type_, _dptr(self.host))
@@ -218,7 +237,12 @@ class Plan:
         """Reference Exec: transform on the device and download into `out`
         (a fresh complex128 array if None)."""
         if out is None:
-            out = np.empty(self.n, dtype=np.complex128)
+            out = np.empty(self.n, dtype=self.dtype)
+        if self.precision is not None:
+            fn = getattr(self.lib, f"tope{self.ndim}DExecEx")
+            fn(C.byref(self.fw.f), C.byref(self.t), out.ctypes.data_as(C.c_void_p), direction)
+            self.fw.check(f"tope{self.ndim}DExecEx")
+            return out
         buf = out.view(np.float64)
         fn = getattr(self.lib, f"tope{self.ndim}DExec")
         fn(C.byref(self.fw.f), C.byref(self.t), _dptr(buf), direction)
@@ -231,12 +255,12 @@ class Plan:
         self.fw.check(f"tope{self.ndim}DExecDevice")
 
     def upload(self, x):
-        h = np.ascontiguousarray(x, dtype=np.complex128).reshape(-1)
+        h = np.ascontiguousarray(x, dtype=self.dtype).reshape(-1)
         self.lib.topePlanUpload(C.byref(self.fw.f), self.impl, h.ctypes.data_as(C.c_void_p), h.nbytes)
         self.fw.check("topePlanUpload")
 
     def download(self):
-        out = np.empty(self.n, dtype=np.complex128)
+        out = np.empty(self.n, dtype=self.dtype)
         self.lib.topePlanDownload(C.byref(self.fw.f), self.impl, out.ctypes.data_as(C.c_void_p), out.nbytes)
         self.fw.check("topePlanDownload")
         return out

@@ -127,6 +127,23 @@ def planted_input(seed, dims, count=8):
     return x, peaks
 
 
+def powerlaw_spectrum(seed, dims, slope=-2.0):
+    """k-space coefficients of a Gaussian random field with power |k|^slope:
+    complex Gaussian(0,1) times |k|^(slope/2) (k = 0 set to 0), |k| from the
+    signed frequency indices.  The real-space field is its inverse DFT."""
+    nx, ny, nz = dims
+    z = complex_gaussian(seed, nx * ny * nz).reshape(nz, ny, nx)
+    fx = np.fft.fftfreq(nx) * nx
+    fy = np.fft.fftfreq(ny) * ny
+    fz = np.fft.fftfreq(nz) * nz
+    for iz in range(nz):
+        k2 = fz[iz] ** 2 + fy[:, None] ** 2 + fx[None, :] ** 2
+        with np.errstate(divide="ignore"):
+            amp = np.where(k2 > 0, k2 ** (slope / 4.0), 0.0)
+        z[iz] *= amp
+    return z.reshape(-1)
+
+
 def interleaved(x):
     """complex128 -> contiguous float64 view (re, im, re, im, ...)."""
     x = np.ascontiguousarray(x, dtype=np.complex128)
