@@ -213,6 +213,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--legs", default="all",
+                    help="comma list of 2d,3d256,mp,many,3d512 (default all; 2d is always run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -237,19 +239,28 @@ def main():
     dom_bytes = 2 * 16 * n2  # one read + one write of the array per pass
     achieved = gbs(dom_bytes, pass2[dom] * 1e-3)
 
+    legs = set(args.legs.split(",")) if args.legs != "all" else {"3d256", "mp", "many", "3d512"}
+    extra = {}
     # ---- 3D 256^3 FP64 (same recipe, beside it) ---------------------------
-    dims3 = (256, 256, 256)
-    n3 = 256 ** 3
-    x3, _ = workloads.planted_input(workloads.SEED_3D, dims3)
-    p3 = topefft.Plan(fw, dims3, x3)
-    t3, pass3 = measure(fw, p3, max(5, args.steps // 2), args.warmup)
-    alg3 = 2 * 16 * n3 * 3
-    p3.destroy()
-    del x3
+    if "3d256" in legs:
+        dims3 = (256, 256, 256)
+        n3 = 256 ** 3
+        x3, _ = workloads.planted_input(workloads.SEED_3D, dims3)
+        p3 = topefft.Plan(fw, dims3, x3)
+        t3, pass3 = measure(fw, p3, max(5, args.steps // 2), args.warmup)
+        alg3 = 2 * 16 * n3 * 3
+        p3.destroy()
+        del x3
+        extra["3d_256cubed"] = {"gbs": round(gbs(alg3, t3), 2), "ms": round(t3 * 1e3, 4),
+                                "transforms_per_s": round(1.0 / t3, 2),
+                                "pass_ms": [round(float(m), 4) for m in pass3]}
 
-    mp = mixed_precision_leg(fw, max(5, args.steps // 2), args.warmup)
-    many = uneven_batch_leg(fw, max(5, args.steps // 2), args.warmup)
-    big3 = large_3d_leg(fw, 4, 1)
+    if "mp" in legs:
+        extra["mixed_precision_batch"] = mixed_precision_leg(fw, max(5, args.steps // 2), args.warmup)
+    if "many" in legs:
+        extra["uneven_batch"] = uneven_batch_leg(fw, max(5, args.steps // 2), args.warmup)
+    if "3d512" in legs:
+        extra["3d_512cubed"] = large_3d_leg(fw, 4, 1)
 
     cpu = None
     if not args.no_cpu:
@@ -279,12 +290,7 @@ def main():
                    "fp64_tflops_5nlogn": round(flops2 / t2 / 1e12, 3),
                    "algorithmic_bytes_per_transform": alg2,
                    "passes": {n: round(float(m), 4) for n, m in zip(names2, pass2)},
-                   "3d_256cubed": {"gbs": round(gbs(alg3, t3), 2), "ms": round(t3 * 1e3, 4),
-                                   "transforms_per_s": round(1.0 / t3, 2),
-                                   "pass_ms": [round(float(m), 4) for m in pass3]},
-                   "mixed_precision_batch": mp,
-                   "uneven_batch": many,
-                   "3d_512cubed": big3},
+                   **extra},
         "roofline": {"bound": "hbm", "kernel": names2[dom], "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": profiled_traffic(names2[dom]),
