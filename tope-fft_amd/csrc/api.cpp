@@ -145,12 +145,19 @@ const AxisVariant *find_variant_named(const char *name)
 }
 
 // Variants measured fastest on MI355X for their (length, layout) at the
-// benchmark shapes (2D 4096^2: rows 84 us vs 89-93 us for the others,
-// 4-column split tiles 114 us vs 140 us for 2-column tiles).
+// benchmark shapes: 2D 4096^2 rows 84 us vs 89-93 us for the others, 4-column
+// split tiles 114 us vs 140 us for 2-column tiles; 3D 256^3 columns 78-83 us
+// with 32-column split tiles vs 83-87 us (16, 64); 3D 512^3 FP64 Z pass
+// 887 us with 16-column split tiles vs 1007 us (8) and 925 us (32), complex64
+// Z 443 us with 32-column split tiles vs 701 us (8).
 const char *const kPreferred[] = {
     "axis_double_n4096_e16_tb1_row_split",
     "axis_double_n4096_e16_tb4_col_split",
     "axis_double_n2048_e16_tb4_col_split",
+    "axis_double_n256_e16_tb32_col_split",
+    "axis_double_n512_e16_tb16_col_split",
+    "axis_float_n512_e16_tb32_col_split",
+    "axis_float_n256_e16_tb32_col_split",
 };
 
 // Variant choice: a measured preference if the line count fills its tile;
