@@ -191,6 +191,22 @@ def large_3d_leg(fw, steps, warmup):
     return res
 
 
+def small_1d_leg(fw, steps, warmup):
+    """1D 2^16 FP64 (latency-bound, two launches of the four-step schedule):
+    device time per transform and transforms/s."""
+    import topefft
+    import workloads
+    n = 1 << 16
+    x = workloads.complex_uniform(workloads.SEED_1D, n)
+    p = topefft.Plan(fw, (n,), x)
+    t, pass_ms = measure(fw, p, max(steps, 50), warmup)
+    names = p.pass_names()
+    p.destroy()
+    return {"workload": "1D C2C FP64 n=65536, device-resident", "us_per_transform": round(t * 1e6, 2),
+            "transforms_per_s": round(1.0 / t, 1),
+            "passes_us": [{"kernel": nm, "us": round(float(m) * 1e3, 2)} for nm, m in zip(names, pass_ms)]}
+
+
 def cpu_baseline(x, dims, budget_s=12.0):
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle_py
@@ -239,8 +255,10 @@ def main():
     dom_bytes = 2 * 16 * n2  # one read + one write of the array per pass
     achieved = gbs(dom_bytes, pass2[dom] * 1e-3)
 
-    legs = set(args.legs.split(",")) if args.legs != "all" else {"3d256", "mp", "many", "3d512"}
+    legs = set(args.legs.split(",")) if args.legs != "all" else {"1d", "3d256", "mp", "many", "3d512"}
     extra = {}
+    if "1d" in legs:
+        extra["1d_65536"] = small_1d_leg(fw, args.steps, args.warmup)
     # ---- 3D 256^3 FP64 (same recipe, beside it) ---------------------------
     if "3d256" in legs:
         dims3 = (256, 256, 256)

@@ -10,6 +10,7 @@
 // axes via a four-step split) enqueued back to back on one HIP stream.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -158,14 +159,18 @@ const char *const kPreferred[] = {
     "axis_double_n512_e16_tb16_col_split",
     "axis_float_n512_e16_tb32_col_split",
     "axis_float_n256_e16_tb32_col_split",
+    "axis_float_n4096_e16_tb1_row_split",
 };
+
+constexpr long long kMinBlocks = 256;  // one block per CU at least, when the shape allows
 
 // Variant choice: a measured preference if the line count fills its tile;
 // otherwise strided tiles take the widest tile that the line count fills
 // (wider tiles give longer contiguous segments per row), preferring the
 // full-complex exchange at equal width.  TOPEFFT_VARIANT_<n>_<row|col>=<name>
 // overrides (used by tuning sweeps).
-const AxisVariant *select_variant(int prec, int n, bool cfast, long long lines, bool dif2 = false)
+const AxisVariant *select_variant(int prec, int n, bool cfast, long long lines, bool dif2 = false,
+                                  long long outer = 1)
 {
     char key[64];
     std::snprintf(key, sizeof key, "TOPEFFT_VARIANT_%d_%s%s", n, cfast ? "col" : "row", dif2 ? "_dif2" : "");
@@ -173,16 +178,36 @@ const AxisVariant *select_variant(int prec, int n, bool cfast, long long lines, 
         const AxisVariant *v = find_variant_named(ov);
         if (v && v->prec == prec && v->n == n && v->dif2 == dif2) return v;
     }
+    // blocks a tile width gives for this shape (small shapes: keep the CUs busy)
+    auto blocks = [&](const AxisVariant *v) { return outer * ((lines + v->tb - 1) / v->tb); };
     for (const char *name : kPreferred) {
         const AxisVariant *v = find_variant_named(name);
-        if (v && v->prec == prec && v->n == n && v->cfast == cfast && v->dif2 == dif2 && v->tb <= lines) return v;
+        if (v && v->prec == prec && v->n == n && v->cfast == cfast && v->dif2 == dif2 && v->tb <= lines &&
+            blocks(v) >= kMinBlocks)
+            return v;
     }
     int cnt = 0;
     const AxisVariant *vs = axis_variants(&cnt);
+    auto matches = [&](const AxisVariant *v) {
+        return v->prec == prec && v->n == n && v->cfast == cfast && v->dif2 == dif2;
+    };
+    // Small shapes: when no tile gives every CU a block, take the tile that
+    // gives the most blocks (full-complex exchange first); otherwise only
+    // tiles that still give every CU a block compete.
+    long long most = 0;
+    for (int i = 0; i < cnt; ++i)
+        if (matches(&vs[i])) most = std::max(most, blocks(&vs[i]));
     const AxisVariant *best = nullptr;
+    if (most > 0 && most < kMinBlocks) {
+        for (int i = 0; i < cnt; ++i) {
+            const AxisVariant *v = &vs[i];
+            if (matches(v) && blocks(v) == most && (!best || (best->split && !v->split))) best = v;
+        }
+        return best;
+    }
     for (int i = 0; i < cnt; ++i) {
         const AxisVariant *v = &vs[i];
-        if (v->prec != prec || v->n != n || v->cfast != cfast || v->dif2 != dif2) continue;
+        if (!matches(v) || blocks(v) < kMinBlocks) continue;
         if (!best) {
             best = v;
             continue;
@@ -293,7 +318,7 @@ class Planner {
     {
         const bool dif2 = pair != 0;
         const bool cfast = !(gi.is == 1 && go.is == 1);
-        const AxisVariant *v = select_variant(P->prec, n, cfast, dif2 ? 2 * gi.C : gi.C, dif2);
+        const AxisVariant *v = select_variant(P->prec, n, cfast, dif2 ? 2 * gi.C : gi.C, dif2, gi.O * reps);
         if (!v) return TOPE_ERROR_NO_KERNEL;
         HostSched h = host_sched(v->n, v->e);
         if (h.twsize != v->twsize) return TOPE_ERROR_NO_KERNEL;  // host/device layouts disagree
