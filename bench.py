@@ -104,7 +104,8 @@ def mixed_precision_leg(fw, steps, warmup):
                     "gbs": round(gbs(2 * 16 * n * count, t64), 1)},
            "complex64": {}}
     x32 = x.astype(np.complex64)
-    for mode in (topefft.MODE_MFMA_BF16X6, topefft.MODE_MFMA_FP32, topefft.MODE_MFMA_BF16X3, topefft.MODE_VALU):
+    for mode in (topefft.MODE_MFMA_FP16X3, topefft.MODE_MFMA_BF16X6, topefft.MODE_MFMA_FP32,
+                 topefft.MODE_MFMA_BF16X3, topefft.MODE_VALU):
         out, t, pm, name = measure_batch(fw, x32, n, count, topefft.PREC_F32, mode, steps, warmup)
         d = out.astype(np.complex128) - ref
         res["complex64"][topefft.MODE_NAMES[mode]] = {

@@ -193,6 +193,8 @@ const char *topePlanPassName(void *plan_impl, int pass);
  *                         into 2 bf16 parts, 3 products (~1e-5 rel-L2)
  *   TOPE_MODE_MFMA_BF16X6 3 bf16 parts, 6 products (~FP32 accuracy)
  *   TOPE_MODE_MFMA_FP32   plain FP32 operands on MFMA
+ *   TOPE_MODE_MFMA_FP16X3 2 fp16 parts, 3 products (~FP32 accuracy), each
+ *                         transform scaled by a power of two into fp16 range
  * MFMA modes take n = 256 or 4096.  Twiddles and DFT-matrix constants are
  * generated in FP64 on the GPU and rounded once; INVERSE scales by 1/n. */
 #define TOPE_PREC_F64 0
@@ -201,6 +203,7 @@ const char *topePlanPassName(void *plan_impl, int pass);
 #define TOPE_MODE_MFMA_BF16X3 1
 #define TOPE_MODE_MFMA_BF16X6 2
 #define TOPE_MODE_MFMA_FP32 3
+#define TOPE_MODE_MFMA_FP16X3 4
 
 struct topePlanBatch {
     int n;

@@ -15,7 +15,7 @@ from conftest import rel_l2
 pytestmark = pytest.mark.gpu
 
 MODE_TOL = {topefft.MODE_VALU: 1e-6, topefft.MODE_MFMA_FP32: 1e-6, topefft.MODE_MFMA_BF16X6: 1e-6,
-            topefft.MODE_MFMA_BF16X3: 1e-4}
+            topefft.MODE_MFMA_BF16X3: 1e-4, topefft.MODE_MFMA_FP16X3: 1e-6}
 
 
 def batch_input(n, count, seed):
@@ -43,7 +43,8 @@ def test_batch_fp64(oracle, n):
 
 
 @pytest.mark.parametrize("mode", [topefft.MODE_VALU, topefft.MODE_MFMA_BF16X3, topefft.MODE_MFMA_BF16X6,
-                                  topefft.MODE_MFMA_FP32], ids=lambda m: topefft.MODE_NAMES[m])
+                                  topefft.MODE_MFMA_FP32, topefft.MODE_MFMA_FP16X3],
+                         ids=lambda m: topefft.MODE_NAMES[m])
 @pytest.mark.parametrize("n", [256, 4096])
 def test_batch_fp32_modes(oracle, n, mode):
     count = 40
@@ -75,6 +76,36 @@ def test_batch_fp32_full_4096(oracle):
         mx = np.max(np.abs(got - ref))
         print(f"4096x4096 mode={topefft.MODE_NAMES[mode]} rel-L2 vs FP64 {err:.3e} max-abs {mx:.3e}")
         assert err < tol
+
+
+@pytest.mark.parametrize("n", [256, 4096])
+def test_batch_fp16x3_range(oracle, n):
+    # fp16 operands: every transform is scaled by its own power of two, so
+    # magnitudes far outside fp16's range, mixed within one batch, keep the
+    # same relative accuracy; zero rows stay zero; one dominant element
+    # beside tiny ones is exact up to FP32 rounding
+    rows = []
+    base = batch_input(n, 6, 300 + n).reshape(6, n)
+    for r, s in enumerate([1e30, 1e-30, 1.0, 0.0, 3e4, 1e-6]):
+        rows.append(base[r] * s)
+    spike = np.full(n, 1e-12 + 1e-12j)
+    spike[n // 3] = 1e20
+    rows.append(spike)
+    x = np.concatenate(rows).astype(np.complex64)
+    count = len(rows)
+    for d in (topefft.FORWARD, topefft.INVERSE):
+        got = run_batch(n, count, x, topefft.PREC_F32, topefft.MODE_MFMA_FP16X3, d).reshape(count, n)
+        got = got.astype(np.complex128)
+        for r in range(count):
+            xr = x[r * n:(r + 1) * n].astype(np.complex128)
+            want = oracle.dft(xr, (n,), d)
+            if not np.any(xr):
+                assert not np.any(got[r]), r
+                continue
+            err = rel_l2(got[r], want)
+            print(f"n={n} dir={d} row {r} rel-L2 {err:.3e}")
+            assert err < 1e-6, (r, err)
+            assert np.all(np.isfinite(got[r]))
 
 
 def test_batch_round_trip_mfma():

@@ -601,7 +601,8 @@ int build_plan(struct topeFFT *f, int shape, long long nx, long long ny, long lo
     if (prec != PREC_F64 && prec != PREC_F32) return TOPE_ERROR_INVALID_ARG;
     if (mode != MP_VALU && (prec != PREC_F32 || shape != SHAPE_BATCH)) return TOPE_ERROR_UNSUPPORTED;
     if (prec == PREC_F32 && shape == 1 && !is_pow2(nx)) return TOPE_ERROR_UNSUPPORTED;
-    if (mode != MP_VALU && mode != MP_BF16X3 && mode != MP_BF16X6 && mode != MP_FP32) return TOPE_ERROR_INVALID_ARG;
+    if (mode != MP_VALU && mode != MP_BF16X3 && mode != MP_BF16X6 && mode != MP_FP32 && mode != MP_FP16X3)
+        return TOPE_ERROR_INVALID_ARG;
     if (mode != MP_VALU && !mfma_supported((int)nx)) return TOPE_ERROR_UNSUPPORTED;
     if (!d) return TOPE_ERROR_INVALID_ARG;
 
@@ -654,7 +655,10 @@ int build_plan(struct topeFFT *f, int shape, long long nx, long long ny, long lo
         ps.m.k = static_cast<const MfmaConsts *>(kc);
         ps.m.scale = 1.0f;
         ps.m.in_elems = ps.m.out_elems = total;
-        ps.name = mode == MP_BF16X3 ? "mfma_rows_bf16x3" : mode == MP_BF16X6 ? "mfma_rows_bf16x6" : "mfma_rows_fp32";
+        ps.name = mode == MP_BF16X3   ? "mfma_rows_bf16x3"
+                  : mode == MP_BF16X6 ? "mfma_rows_bf16x6"
+                  : mode == MP_FP16X3 ? "mfma_rows_fp16x3"
+                                      : "mfma_rows_fp32";
         P->passes.push_back(ps);
         P->axis_tw[0].n = (int)nx;
         P->axis_tw[0].e = 16;

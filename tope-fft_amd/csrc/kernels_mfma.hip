@@ -1,5 +1,7 @@
 // Instantiations of the MFMA row kernel and the FP64 generation of its
 // DFT-16 operand constants.
+#include <cstdlib>
+
 #include "internal.hpp"
 #include "mfma_kernel.hpp"
 
@@ -43,6 +45,20 @@ __global__ void gen_mfma_consts(MfmaConsts *K)
             dst[slot] = h;
         }
     }
+    // f16 form: the bf16 slot layout, two fp16 parts (round to nearest)
+    {
+        const int m = 4 * g + (slot & 3);
+        double s, c;
+        sincospi(2.0 * (double)((i * m) & 15) / 16.0, &s, &c);
+        const double wr = c, wi = sign * s;
+        const double a = ri == 0 ? (slot < 4 ? wr : -wi) : (slot < 4 ? wi : wr);
+        const _Float16 h0 = (_Float16)(float)a;
+        const _Float16 h1 = (_Float16)(float)(a - (double)(float)h0);
+        _Float16 *d0 = reinterpret_cast<_Float16 *>(&K->h[dir][ri][0][lane][0]);
+        _Float16 *d1 = reinterpret_cast<_Float16 *>(&K->h[dir][ri][1][lane][0]);
+        d0[slot] = h0;
+        d1[slot] = h1;
+    }
     // f32 form: instruction kk = slot, lane group g supplies K = 4kk + g
     {
         const int kk = slot;
@@ -65,19 +81,39 @@ hipError_t gen_mfma_constants(void *out, hipStream_t s)
 
 size_t mfma_constants_bytes() { return sizeof(MfmaConsts); }
 
-template <int MODE, int N>
-static hipError_t launch_n(const void *in, void *out, const MfmaArgs &a, int sign, hipStream_t s)
+template <int MODE, int N, int W>
+static hipError_t launch_nw(const void *in, void *out, const MfmaArgs &a, int sign, hipStream_t s)
 {
-    using SH = MfmaShape<MODE, N, -1>;
+    using SH = MfmaShape<N, W>;
     const long long blocks = (a.count + SH::TPB - 1) / SH::TPB;
     if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
     const cpx<float> *i = static_cast<const cpx<float> *>(in);
     cpx<float> *o = static_cast<cpx<float> *>(out);
     if (sign < 0)
-        hipLaunchKernelGGL((mfma_rows_kernel<MODE, N, -1>), dim3((unsigned)blocks), dim3(256), 0, s, i, o, a);
+        hipLaunchKernelGGL((mfma_rows_kernel<MODE, N, -1, W>), dim3((unsigned)blocks), dim3(SH::THREADS), 0, s, i,
+                           o, a);
     else
-        hipLaunchKernelGGL((mfma_rows_kernel<MODE, N, 1>), dim3((unsigned)blocks), dim3(256), 0, s, i, o, a);
+        hipLaunchKernelGGL((mfma_rows_kernel<MODE, N, 1, W>), dim3((unsigned)blocks), dim3(SH::THREADS), 0, s, i,
+                           o, a);
     return hipGetLastError();
+}
+
+// waves per transform: 4 (4 groups per lane, the measured default) or 8
+// (2 groups per lane), TOPEFFT_MFMA_WAVES=8 selects the latter
+static int mfma_waves()
+{
+    static const int w = [] {
+        const char *e = std::getenv("TOPEFFT_MFMA_WAVES");
+        return e && std::atoi(e) == 8 ? 8 : 4;
+    }();
+    return w;
+}
+
+template <int MODE, int N>
+static hipError_t launch_n(const void *in, void *out, const MfmaArgs &a, int sign, hipStream_t s)
+{
+    if (mfma_waves() == 4) return launch_nw<MODE, N, 4>(in, out, a, sign, s);
+    return launch_nw<MODE, N, 8>(in, out, a, sign, s);
 }
 
 template <int MODE>
@@ -98,6 +134,7 @@ hipError_t launch_mfma_rows(int mode, int n, const void *in, void *out, const Mf
     case MP_BF16X3: return launch_mode<MP_BF16X3>(n, in, out, a, sign, s);
     case MP_BF16X6: return launch_mode<MP_BF16X6>(n, in, out, a, sign, s);
     case MP_FP32: return launch_mode<MP_FP32>(n, in, out, a, sign, s);
+    case MP_FP16X3: return launch_mode<MP_FP16X3>(n, in, out, a, sign, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -17,6 +17,18 @@
 //   MP_BF16X6  x = x0 + x1 + x2 (exact for FP32 x), F = F0 + F1 + F2; the six
 //              products with i + j <= 2, dropped terms ~2^-24.
 //   MP_FP32    plain FP32 operands on v_mfma_f32_16x16x4_f32.
+//   MP_FP16X3  x = x0 + x1 (fp16 round-to-nearest parts, the residual taken
+//              exactly in FP32; |x - x0 - x1| <= 2^-22 |x| in the normal
+//              range), F = F0 + F1 (fp16 parts from FP64); products
+//              F1x0 + F0x1 + F0x0 on v_mfma_f32_16x16x32_f16.  fp16 has
+//              5 exponent bits, so each transform is scaled by a power of
+//              two 2^k, chosen from the largest |Re|, |Im| of its input,
+//              so that the input lies below 2^7: a radix-16 stage grows the
+//              largest magnitude by at most 16, and the last stage's input
+//              stays below 2^15 < 65504 (N = 16^3).  Elements smaller than
+//              2^-14 of the scaled range keep an absolute error <= 2^-25
+//              there (fp16 subnormal spacing).  The output is scaled back
+//              by 2^-k, exactly.
 //
 // Lane layout.  Lane l of a wave handles butterfly c = l % 16 of a group of
 // 16 and holds its inputs m = 4g .. 4g+3 (g = l / 16), real and imaginary
@@ -34,9 +46,11 @@
 
 namespace tope {
 
-enum { MP_VALU = 0, MP_BF16X3 = 1, MP_BF16X6 = 2, MP_FP32 = 3 };
+enum { MP_VALU = 0, MP_BF16X3 = 1, MP_BF16X6 = 2, MP_FP32 = 3, MP_FP16X3 = 4 };
 
 typedef __bf16 tope_bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 tope_f16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 tope_f16x2 __attribute__((ext_vector_type(2)));
 typedef float tope_f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int tope_u32x4v __attribute__((ext_vector_type(4)));
 
@@ -44,9 +58,11 @@ typedef unsigned int tope_u32x4v __attribute__((ext_vector_type(4)));
 // rounded once.  For each direction (0 forward, 1 inverse):
 //   bf16: part p (0..2) of A_re and A_im, 4 dwords (8 bf16) per lane
 //   f32:  A_re, A_im for the 8 instructions, one float per lane each
+//   f16:  part p (0..1) of A_re and A_im, same slot layout as bf16
 struct MfmaConsts {
     unsigned int bf[2][2][3][64][4];  // [dir][re/im][part][lane][dword]
     float f32[2][2][8][64];           // [dir][re/im][kk][lane]
+    unsigned int h[2][2][2][64][4];   // [dir][re/im][part][lane][dword]
 };
 
 struct MfmaArgs {
@@ -93,11 +109,35 @@ __device__ __forceinline__ tope_f32x4 mfma_bf16(tope_u32x4v a, tope_u32x4v b, to
                                                    __builtin_bit_cast(tope_bf16x8, b), c, 0, 0, 0);
 }
 
+__device__ __forceinline__ tope_f32x4 mfma_f16(tope_u32x4v a, tope_u32x4v b, tope_f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(tope_f16x8, a),
+                                                  __builtin_bit_cast(tope_f16x8, b), c, 0, 0, 0);
+}
+
+// Split the lane's 8 real operands into two fp16 parts, packed two per
+// dword in slot order: x0 = fp16(x) (round to nearest), x1 = fp16(x - x0);
+// x - x0 is exact in FP32 (x0 lies within a factor of two of x, or is 0).
+__device__ __forceinline__ void split_f16(const float (&x)[8], tope_u32x4v &p0, tope_u32x4v &p1)
+{
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        float a = x[2 * d], b = x[2 * d + 1];
+        // keep a, b as FP32 values: otherwise the producing multiply-add is
+        // fused into the fp16 conversion and then computed twice
+        asm("" : "+v"(a), "+v"(b));
+        const tope_f16x2 h = {(_Float16)a, (_Float16)b};
+        const tope_f16x2 l = {(_Float16)(a - (float)h.x), (_Float16)(b - (float)h.y)};
+        p0[d] = __builtin_bit_cast(unsigned int, h);
+        p1[d] = __builtin_bit_cast(unsigned int, l);
+    }
+}
+
 // One radix-16 DFT on MFMA for the 16 butterflies of this wave's group.
 // in: x[s] = Re/Im of the lane's 4 inputs (slots 0..3 real, 4..7 imaginary);
-// out: y[r] = output 4g + r (complex).
+// out: y[r] = Re of output 4g + r, y[4 + r] = its Im (the same slot layout).
 template <int MODE>
-__device__ __forceinline__ void dft16_mfma(const float (&x)[8], cpx<float> (&y)[4], const MfmaConsts *__restrict__ K,
+__device__ __forceinline__ void dft16_mfma(const float (&x)[8], float (&y)[8], const MfmaConsts *__restrict__ K,
                                            int dir, int lane)
 {
     tope_f32x4 ar = {0, 0, 0, 0}, ai = {0, 0, 0, 0};
@@ -108,6 +148,24 @@ __device__ __forceinline__ void dft16_mfma(const float (&x)[8], cpx<float> (&y)[
             ar = __builtin_amdgcn_mfma_f32_16x16x4f32(K->f32[dir][0][kk][lane], b, ar, 0, 0, 0);
             ai = __builtin_amdgcn_mfma_f32_16x16x4f32(K->f32[dir][1][kk][lane], b, ai, 0, 0, 0);
         }
+    } else if constexpr (MODE == MP_FP16X3) {
+        tope_u32x4v x0, x1;
+        split_f16(x, x0, x1);
+        tope_u32x4v fr[2], fi[2];
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const unsigned int *q = K->h[dir][0][p][lane];
+            const unsigned int *w = K->h[dir][1][p][lane];
+            fr[p] = tope_u32x4v{q[0], q[1], q[2], q[3]};
+            fi[p] = tope_u32x4v{w[0], w[1], w[2], w[3]};
+        }
+        // small terms first
+        ar = mfma_f16(fr[1], x0, ar);
+        ai = mfma_f16(fi[1], x0, ai);
+        ar = mfma_f16(fr[0], x1, ar);
+        ai = mfma_f16(fi[0], x1, ai);
+        ar = mfma_f16(fr[0], x0, ar);
+        ai = mfma_f16(fi[0], x0, ai);
     } else {
         constexpr int PARTS = MODE == MP_BF16X6 ? 3 : 2;
         tope_u32x4v xp[PARTS];
@@ -131,34 +189,57 @@ __device__ __forceinline__ void dft16_mfma(const float (&x)[8], cpx<float> (&y)[
             }
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) y[r] = cpx<float>{ar[r], ai[r]};
+    for (int r = 0; r < 4; ++r) {
+        y[r] = ar[r];
+        y[4 + r] = ai[r];
+    }
 }
 
-// LDS index of element i of block-local transform t (one pad slot per 16).
-template <int N>
-__device__ __forceinline__ int mp_lds(int t, int i) { return t * (N + N / 16) + i + (i >> 4); }
+// LDS word of element i (i < 4096) within a transform's slab: an XOR
+// swizzle of i's low six bits by bits 5, 6, 7, 10, 11 (a bijection), chosen
+// so that every exchange access of the kernel below -- stage-0 outputs
+// (16 j + 4 g + r), stage-1 outputs ((j/16) 256 + j%16 + 16 (4 g + r)) and
+// stage inputs (j + NB (4 g + s)) -- puts the 64 lanes of a wave on 64
+// distinct banks (and each half wave on 32).  The swizzle is linear over
+// GF(2), and bits 8, 9 and 12 up pass through unchanged, so fields in those
+// bits can be added afterwards.
+__device__ __forceinline__ constexpr int mp_swz(int i)
+{
+    return i ^ (((i >> 5) & 1) * 0x01) ^ (((i >> 6) & 1) * 0x12) ^ (((i >> 7) & 1) * 0x28) ^
+           (((i >> 10) & 1) * 0x10) ^ (((i >> 11) & 1) * 0x20);
+}
 
-// One block = TPB transforms of length N = 16^NST, WPT waves per transform,
-// GPW groups of 16 butterflies per wave per stage.
-template <int MODE, int N, int SIGN>
+// One block = TPB transforms of length N = 16^NST, WPT <= WMAX waves per
+// transform, GPW groups of 16 butterflies per wave per stage; blocks hold at
+// least 4 waves.
+template <int N, int WMAX>
 struct MfmaShape {
     static constexpr int NST = ilog2c(N) / 4;
     static constexpr int NB = N / 16;                    // butterflies per stage
     static constexpr int NG = NB / 16;                   // groups per stage
-    static constexpr int WPT = NG >= 4 ? 4 : NG;         // waves per transform
+    static constexpr int WPT = NG >= WMAX ? WMAX : NG;   // waves per transform
     static constexpr int GPW = NG / WPT;                 // groups per wave
-    static constexpr int TPB = 4 / WPT;                  // transforms per block (4 waves)
-    static constexpr int THREADS = 256;
+    static constexpr int TPB = WPT >= 4 ? 1 : 4 / WPT;   // transforms per block
+    static constexpr int THREADS = 64 * WPT * TPB;
     static_assert(N == (1 << (4 * NST)) && NST >= 2, "N must be 16^s, s >= 2");
 };
 
-template <int MODE, int N, int SIGN>
-__global__ void __launch_bounds__(256) mfma_rows_kernel(const cpx<float> *in, cpx<float> *out, const MfmaArgs a)
+// Stockham data flow as in the FP64 axis kernel: stage S (span Ns = 16^S)
+// reads butterfly j's input m at j + NB m and writes its output r to
+// (j / Ns) Ns 16 + (j mod Ns) + r Ns.  Between stages the outputs pass
+// through LDS one component at a time (real parts, then imaginary parts),
+// which halves the LDS a block holds.  Each group's inputs become its
+// outputs in place, so a lane holds 8 GPW floats of transform data.
+template <int MODE, int N, int SIGN, int WMAX>
+__global__ void __launch_bounds__((MfmaShape<N, WMAX>::THREADS)) __attribute__((amdgpu_waves_per_eu(WMAX == 8 ? 8 : 1))) mfma_rows_kernel(const cpx<float> *in,
+                                                                               cpx<float> *out, const MfmaArgs a)
 {
-    using SH = MfmaShape<MODE, N, SIGN>;
+    using SH = MfmaShape<N, WMAX>;
     constexpr int NB = SH::NB, WPT = SH::WPT, GPW = SH::GPW, TPB = SH::TPB, NST = SH::NST;
     using SC = Sched<N, 16>;
-    __shared__ cpx<float> lds[TPB * (N + N / 16)];
+    __shared__ float lds[TPB * N];
+    __shared__ float wave_max[SH::THREADS / 64];
+    static_assert(N <= 4096, "the LDS swizzle covers 12 index bits");
 
     const int tid = threadIdx.x;
     // wave index made provably uniform: buffer descriptors derived from it
@@ -187,10 +268,37 @@ __global__ void __launch_bounds__(256) mfma_rows_kernel(const cpx<float> *in, cp
         }
     }
 
+    float out_scale = a.scale;
+    if constexpr (MODE == MP_FP16X3) {
+        // power-of-two range scale of this transform (all its waves agree)
+        float m = 0.0f;
+#pragma unroll
+        for (int q = 0; q < GPW; ++q)
+#pragma unroll
+            for (int s = 0; s < 8; ++s) m = fmaxf(m, fabsf(x[q][s]));
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+        if constexpr (WPT > 1) {
+            if (lane == 0) wave_max[wave] = m;
+            __syncthreads();
+#pragma unroll
+            for (int w = 0; w < WPT; ++w) m = fmaxf(m, wave_max[tb * WPT + w]);
+        }
+        // m = f 2^e, f in [0.5, 1): the input times 2^(7-e) lies below 2^7;
+        // e is clamped so that both scale factors are normal floats
+        int e = __builtin_amdgcn_frexp_expf(m);
+        e = e < -119 ? -119 : (e > 126 ? 126 : e);
+        const float up = __builtin_ldexpf(1.0f, 7 - e);
+        out_scale *= __builtin_ldexpf(1.0f, e - 7);
+#pragma unroll
+        for (int q = 0; q < GPW; ++q)
+#pragma unroll
+            for (int s = 0; s < 8; ++s) x[q][s] *= up;
+    }
+
 #pragma unroll
     for (int S = 0; S < NST; ++S) {
         const int Ns = 1 << (4 * S);
-        cpx<float> y[GPW][4];
 #pragma unroll
         for (int q = 0; q < GPW; ++q) {
             const int j = (wt * GPW + q) * 16 + c;
@@ -210,38 +318,54 @@ __global__ void __launch_bounds__(256) mfma_rows_kernel(const cpx<float> *in, cp
                     x[q][4 + s] = v.y;
                 }
             }
-            dft16_mfma<MODE>(x[q], y[q], a.k, dir, lane);
+            float y[8];
+            dft16_mfma<MODE>(x[q], y, a.k, dir, lane);
+#pragma unroll
+            for (int s = 0; s < 8; ++s) x[q][s] = y[s];
         }
         if (S == NST - 1) {
             if (valid) {
                 const __amdgpu_buffer_rsrc_t rout = make_rsrc(out + row * N, (a.out_elems - row * N) * CB);
+                // the forward transform of the unscaled modes stores as is
+                constexpr bool SCALED = MODE == MP_FP16X3 || SIGN > 0;
 #pragma unroll
                 for (int q = 0; q < GPW; ++q) {
                     const int j = (wt * GPW + q) * 16 + c;
 #pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        buf_store<float>(scale(y[q][r], a.scale), rout, (j + NB * (4 * g + r)) * CB, 0);
+                    for (int r = 0; r < 4; ++r) {
+                        cpx<float> v{x[q][r], x[q][4 + r]};
+                        if constexpr (SCALED) v = scale(v, out_scale);
+                        buf_store<float>(v, rout, (j + NB * (4 * g + r)) * CB, 0);
+                    }
                 }
             }
         } else {
-            if (S > 0) __syncthreads();
+            // Write index of output r of group q: dst = base_q + (4g + r) Ns,
+            // where base_q + 4g Ns and r Ns occupy disjoint bits, so
+            // swz(dst) = swz(base_q + 4g Ns) ^ swz(r Ns).  Read index of input s:
+            // src = j + NB (4g + s) = (j + 4g NB) | s NB with s NB in bits
+            // 8..11 -- bits 8, 9 pass through (additive) and bits 10, 11
+            // (N = 4096) contribute a constant XOR.
+            float *slab = lds + tb * N;
+            int wbase[GPW], rbase[GPW];
 #pragma unroll
             for (int q = 0; q < GPW; ++q) {
                 const int j = (wt * GPW + q) * 16 + c;
-                const int base = (j / Ns) * Ns * 16 + (j & (Ns - 1));
-#pragma unroll
-                for (int r = 0; r < 4; ++r) lds[mp_lds<N>(tb, base + (4 * g + r) * Ns)] = y[q][r];
+                wbase[q] = mp_swz((j / Ns) * Ns * 16 + (j & (Ns - 1)) + 4 * g * Ns);
+                rbase[q] = mp_swz(j + NB * 4 * g);
             }
-            __syncthreads();
 #pragma unroll
-            for (int q = 0; q < GPW; ++q) {
-                const int j = (wt * GPW + q) * 16 + c;
+            for (int part = 0; part < 2; ++part) {
+                if (S > 0 || part > 0) __syncthreads();  // the previous reads of lds are done
 #pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    const cpx<float> v = lds[mp_lds<N>(tb, j + NB * (4 * g + s))];
-                    x[q][s] = v.x;
-                    x[q][4 + s] = v.y;
-                }
+                for (int q = 0; q < GPW; ++q)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) slab[wbase[q] ^ mp_swz(r * Ns)] = x[q][4 * part + r];
+                __syncthreads();
+#pragma unroll
+                for (int q = 0; q < GPW; ++q)
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) x[q][4 * part + s] = slab[rbase[q] ^ mp_swz(NB * s)];
             }
         }
     }

@@ -71,9 +71,9 @@ MANY_PERSISTENT, MANY_PERSISTENT_GRAPH, MANY_PER_TRANSFORM, MANY_PER_TRANSFORM_G
 MANY_METHOD_NAMES = {0: "persistent", 1: "persistent_graph", 2: "per_transform", 3: "per_transform_graph"}
 
 PREC_F64, PREC_F32 = 0, 1
-MODE_VALU, MODE_MFMA_BF16X3, MODE_MFMA_BF16X6, MODE_MFMA_FP32 = 0, 1, 2, 3
+MODE_VALU, MODE_MFMA_BF16X3, MODE_MFMA_BF16X6, MODE_MFMA_FP32, MODE_MFMA_FP16X3 = 0, 1, 2, 3, 4
 MODE_NAMES = {MODE_VALU: "valu", MODE_MFMA_BF16X3: "mfma_bf16x3", MODE_MFMA_BF16X6: "mfma_bf16x6",
-              MODE_MFMA_FP32: "mfma_fp32"}
+              MODE_MFMA_FP32: "mfma_fp32", MODE_MFMA_FP16X3: "mfma_fp16x3"}
 
 # name -> (restype, argtypes)
 _F, _P1, _P2, _P3 = C.POINTER(TopeFFT), C.POINTER(TopePlan1D), C.POINTER(TopePlan2D), C.POINTER(TopePlan3D)
