@@ -116,8 +116,11 @@ __device__ __forceinline__ tope_f32x4 mfma_f16(tope_u32x4v a, tope_u32x4v b, top
 }
 
 // Split the lane's 8 real operands into two fp16 parts, packed two per
-// dword in slot order: x0 = fp16(x) (round to nearest), x1 = fp16(x - x0);
-// x - x0 is exact in FP32 (x0 lies within a factor of two of x, or is 0).
+// dword in slot order: x0 = fp16(x) (round to nearest), x1 = fp16(x - x0).
+// x - x0 is exact in FP32 (x0 lies within a factor of two of x, or is 0); it
+// is formed by one v_fma_mix_f32 per value, -x0 * 1 + x with x0 read as an
+// fp16 half of the packed dword (bit-identical to converting x0 to FP32 and
+// subtracting, fp16 subnormals included).
 __device__ __forceinline__ void split_f16(const float (&x)[8], tope_u32x4v &p0, tope_u32x4v &p1)
 {
 #pragma unroll
@@ -127,8 +130,12 @@ __device__ __forceinline__ void split_f16(const float (&x)[8], tope_u32x4v &p0, 
         // fused into the fp16 conversion and then computed twice
         asm("" : "+v"(a), "+v"(b));
         const tope_f16x2 h = {(_Float16)a, (_Float16)b};
-        const tope_f16x2 l = {(_Float16)(a - (float)h.x), (_Float16)(b - (float)h.y)};
-        p0[d] = __builtin_bit_cast(unsigned int, h);
+        const unsigned int hb = __builtin_bit_cast(unsigned int, h);
+        float ra, rb;
+        asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(ra) : "v"(hb), "v"(a));
+        asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(rb) : "v"(hb), "v"(b));
+        const tope_f16x2 l = {(_Float16)ra, (_Float16)rb};
+        p0[d] = hb;
         p1[d] = __builtin_bit_cast(unsigned int, l);
     }
 }
@@ -342,30 +349,36 @@ __global__ void __launch_bounds__((MfmaShape<N, WMAX>::THREADS)) __attribute__((
         } else {
             // Write index of output r of group q: dst = base_q + (4g + r) Ns,
             // where base_q + 4g Ns and r Ns occupy disjoint bits, so
-            // swz(dst) = swz(base_q + 4g Ns) ^ swz(r Ns).  Read index of input s:
-            // src = j + NB (4g + s) = (j + 4g NB) | s NB with s NB in bits
-            // 8..11 -- bits 8, 9 pass through (additive) and bits 10, 11
-            // (N = 4096) contribute a constant XOR.
+            // swz(dst) = swz(base_q + 4g Ns) ^ swz(r Ns); q enters base_q
+            // as q * 256 (bits 8, 9: unswizzled), an additive offset.
+            // Read index of input s: src = (j + 4g NB) | s NB; for NB = 256
+            // s NB lies in bits 8, 9 (additive), otherwise it is XORed in.
             float *slab = lds + tb * N;
-            int wbase[GPW], rbase[GPW];
+            static_assert(GPW * 256 <= 1024, "group index must stay in bits 8, 9");
+            const int j0 = wt * GPW * 16 + c;
+            int wadr[4], radr[GPW];
+            {
+                const int wb = mp_swz((j0 / Ns) * Ns * 16 + (j0 & (Ns - 1)) + 4 * g * Ns);
 #pragma unroll
-            for (int q = 0; q < GPW; ++q) {
-                const int j = (wt * GPW + q) * 16 + c;
-                wbase[q] = mp_swz((j / Ns) * Ns * 16 + (j & (Ns - 1)) + 4 * g * Ns);
-                rbase[q] = mp_swz(j + NB * 4 * g);
+                for (int r = 0; r < 4; ++r) wadr[r] = wb ^ mp_swz(r * Ns);
             }
+#pragma unroll
+            for (int q = 0; q < GPW; ++q) radr[q] = mp_swz(j0 + q * 16 + NB * 4 * g);
 #pragma unroll
             for (int part = 0; part < 2; ++part) {
                 if (S > 0 || part > 0) __syncthreads();  // the previous reads of lds are done
 #pragma unroll
                 for (int q = 0; q < GPW; ++q)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) slab[wbase[q] ^ mp_swz(r * Ns)] = x[q][4 * part + r];
+                    for (int r = 0; r < 4; ++r) slab[wadr[r] + q * 256] = x[q][4 * part + r];
                 __syncthreads();
 #pragma unroll
                 for (int q = 0; q < GPW; ++q)
 #pragma unroll
-                    for (int s = 0; s < 4; ++s) x[q][4 * part + s] = slab[rbase[q] ^ mp_swz(NB * s)];
+                    for (int s = 0; s < 4; ++s) {
+                        const int idx = NB == 256 ? radr[q] + NB * s : radr[q] ^ mp_swz(NB * s);
+                        x[q][4 * part + s] = slab[idx];
+                    }
             }
         }
     }
