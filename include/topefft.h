@@ -233,6 +233,16 @@ void tope2DPlanInitEx(struct topeFFT *f, struct topePlan2D *t, int x, int y, int
                       const void *host);
 void tope3DPlanInitEx(struct topeFFT *f, struct topePlan3D *t, int x, int y, int z, int type, int precision,
                       const void *host);
+/* Complex64 2D/3D plans whose axis passes run on the matrix cores in the
+ * given TOPE_MODE_MFMA_* operand mode where a kernel covers the axis:
+ * contiguous rows of 256 or 4096, unit-stride column axes of 256, 512 or
+ * 1024 with a multiple of 16 lines; other axes take the FP32 VALU pass
+ * (topePlanPassName tells which).  precision must be TOPE_PREC_F32 unless
+ * mode is TOPE_MODE_VALU (then identical to the Ex functions). */
+void tope2DPlanInitMode(struct topeFFT *f, struct topePlan2D *t, int x, int y, int type, int precision, int mode,
+                        const void *host);
+void tope3DPlanInitMode(struct topeFFT *f, struct topePlan3D *t, int x, int y, int z, int type, int precision,
+                        int mode, const void *host);
 void tope2DExecEx(struct topeFFT *f, struct topePlan2D *t, void *host, int dir);
 void tope3DExecEx(struct topeFFT *f, struct topePlan3D *t, void *host, int dir);
 

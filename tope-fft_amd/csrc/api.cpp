@@ -24,6 +24,7 @@
 #include "../../include/topefft.h"
 #include "internal.hpp"
 #include "mfma_kernel.hpp"
+#include "mfma_tile.hpp"
 
 using namespace tope;
 
@@ -43,7 +44,7 @@ struct FrameworkImpl {
     EventPair last;  // last exec's device time (profileThis)
 };
 
-enum PassKind { PASS_AXIS = 0, PASS_DFT = 1, PASS_MFMA = 2 };
+enum PassKind { PASS_AXIS = 0, PASS_DFT = 1, PASS_MFMA = 2, PASS_MFMA_TILE = 3 };
 
 struct Pass {
     int kind = PASS_AXIS;
@@ -51,6 +52,7 @@ struct Pass {
     AxisArgs<double> a{};              // FP64 axis pass
     AxisArgs<float> a32{};             // FP32 axis pass
     MfmaArgs m{};                      // PASS_MFMA
+    MfmaTileArgs mt{};                 // PASS_MFMA_TILE
     int mfma_mode = 0, mfma_n = 0;
     unsigned grid = 0;
     int in_buf = 0, out_buf = 0;
@@ -443,6 +445,82 @@ class Planner {
         return TOPE_SUCCESS;
     }
 
+    // DFT-16 operand constants of the MFMA passes (one table per plan).
+    hipError_t mfma_consts(const MfmaConsts **out)
+    {
+        if (!kc_) {
+            void *d = nullptr;
+            hipError_t err = hipMalloc(&d, mfma_constants_bytes());
+            if (err != hipSuccess) return err;
+            P->allocs.push_back(d);
+            err = gen_mfma_constants(d, st);
+            if (err != hipSuccess) return err;
+            kc_ = static_cast<const MfmaConsts *>(d);
+        }
+        *out = kc_;
+        return hipSuccess;
+    }
+
+    // Axis pass on the matrix cores when a kernel covers the geometry
+    // (complex64 storage): contiguous rows of n = 256 or 4096 back to back
+    // (row kernel), or unit-stride column tiles of 16 lines with n = 256,
+    // 512, 1024 (tile kernel).  Otherwise the FP32 VALU axis pass.
+    int axis_mp(long long n, const Geo &g, int *cur, int axis_id, int mode)
+    {
+        if (mode == MP_VALU || n < 256 || n > 4096) return axis(n, g, cur, axis_id);
+        const bool rows = g.is == 1 && g.cs == n && (g.O == 1 || g.os == g.C * n) && mfma_supported((int)n);
+        const bool cols = g.cs == 1 && g.C % 16 == 0 && mfma_tile_supported((int)n) &&
+                          (15 + (n - 1) * g.is) * (long long)P->elem < 0x7fffffffLL;
+        if (!rows && !cols) return axis(n, g, cur, axis_id);
+        void *tw = nullptr;
+        hipError_t err = stage_table((int)n, 16, &tw);
+        const MfmaConsts *kc = nullptr;
+        if (err == hipSuccess) err = mfma_consts(&kc);
+        if (err != hipSuccess) return map_err(err);
+        if (axis_id >= 0 && !P->axis_tw[axis_id].table) {
+            P->axis_tw[axis_id].n = (int)n;
+            P->axis_tw[axis_id].e = 16;
+            P->axis_tw[axis_id].table = tw;
+        }
+        Pass ps;
+        ps.mfma_mode = mode;
+        ps.mfma_n = (int)n;
+        ps.in_buf = ps.out_buf = *cur;
+        if (rows) {
+            ps.kind = PASS_MFMA;
+            ps.m.count = g.O * g.C;
+            ps.m.tw = static_cast<const cpx<float> *>(tw);
+            ps.m.k = kc;
+            ps.m.scale = 1.0f;
+            ps.m.in_elems = ps.m.out_elems = P->total;
+            ps.name = mode == MP_BF16X3   ? "mfma_rows_bf16x3"
+                      : mode == MP_BF16X6 ? "mfma_rows_bf16x6"
+                      : mode == MP_FP16X3 ? "mfma_rows_fp16x3"
+                                          : "mfma_rows_fp32";
+        } else {
+            ps.kind = PASS_MFMA_TILE;
+            MfmaTileArgs &t = ps.mt;
+            t.O = g.O;
+            t.os = g.os;
+            t.tiles = g.C / 16;
+            t.is = g.is;
+            t.tw = static_cast<const cpx<float> *>(tw);
+            t.k = kc;
+            t.scale = 1.0f;
+            t.in_elems = t.out_elems = P->total;
+            t.xcd = std::getenv("TOPEFFT_NO_XCD_REMAP") ? 0 : 1;
+            const long long grid = g.O * t.tiles;
+            if (grid <= 0 || grid > 0x7fffffffLL) return TOPE_ERROR_INVALID_SIZE;
+            ps.grid = (unsigned)grid;
+            ps.name = mode == MP_BF16X3   ? "mfma_cols_bf16x3"
+                      : mode == MP_BF16X6 ? "mfma_cols_bf16x6"
+                      : mode == MP_FP16X3 ? "mfma_cols_fp16x3"
+                                          : "mfma_cols_fp32";
+        }
+        P->passes.push_back(ps);
+        return TOPE_SUCCESS;
+    }
+
     // 2D with a long Y axis (ny <= 4096): the X pass takes rows y and
     // y + ny/2 together and ends with the radix-2 DIF step of Y between
     // them; the Y pass then runs two independent ny/2-point transforms per
@@ -469,6 +547,7 @@ class Planner {
    private:
     PlanImpl *P;
     hipStream_t st;
+    const MfmaConsts *kc_ = nullptr;
 };
 
 // 2D plans whose Y axis is at least this long (and fits one pass) use
@@ -542,6 +621,11 @@ int launch_pass(PlanImpl *P, size_t i, hipStream_t s, int dir)
         m.scale = ps.last ? (float)sc : 1.0f;  // 1/N is a power of two: exact in FP32
         hipError_t err = launch_mfma_rows(ps.mfma_mode, ps.mfma_n, ib, ob, m, sign, s);
         if (err != hipSuccess) return map_err(err);
+    } else if (ps.kind == PASS_MFMA_TILE) {
+        MfmaTileArgs m = ps.mt;
+        m.scale = ps.last ? (float)sc : 1.0f;
+        hipError_t err = launch_mfma_tile(ps.mfma_mode, ps.mfma_n, ib, ob, m, ps.grid, sign, s);
+        if (err != hipSuccess) return map_err(err);
     } else {
         hipError_t err = launch_naive_dft(P->prec, ib, ob, ps.dft_roots, ps.dft_n, sign, ps.last ? sc : 1.0, s);
         if (err != hipSuccess) return map_err(err);
@@ -599,11 +683,12 @@ int build_plan(struct topeFFT *f, int shape, long long nx, long long ny, long lo
     if ((shape == 2 || shape == 3) && (!is_pow2(nx) || !is_pow2(ny) || !is_pow2(nz))) return TOPE_ERROR_INVALID_SIZE;
     if (shape == SHAPE_BATCH && !is_pow2(nx)) return TOPE_ERROR_INVALID_SIZE;
     if (prec != PREC_F64 && prec != PREC_F32) return TOPE_ERROR_INVALID_ARG;
-    if (mode != MP_VALU && (prec != PREC_F32 || shape != SHAPE_BATCH)) return TOPE_ERROR_UNSUPPORTED;
+    if (mode != MP_VALU && (prec != PREC_F32 || (shape != SHAPE_BATCH && shape != 2 && shape != 3)))
+        return TOPE_ERROR_UNSUPPORTED;
     if (prec == PREC_F32 && shape == 1 && !is_pow2(nx)) return TOPE_ERROR_UNSUPPORTED;
     if (mode != MP_VALU && mode != MP_BF16X3 && mode != MP_BF16X6 && mode != MP_FP32 && mode != MP_FP16X3)
         return TOPE_ERROR_INVALID_ARG;
-    if (mode != MP_VALU && !mfma_supported((int)nx)) return TOPE_ERROR_UNSUPPORTED;
+    if (mode != MP_VALU && shape == SHAPE_BATCH && !mfma_supported((int)nx)) return TOPE_ERROR_UNSUPPORTED;
     if (!d) return TOPE_ERROR_INVALID_ARG;
 
     PlanImpl *P = new (std::nothrow) PlanImpl();
@@ -685,6 +770,11 @@ int build_plan(struct topeFFT *f, int shape, long long nx, long long ny, long lo
         P->axis_tw[0].n = (int)nx;
         P->axis_tw[0].e = 0;
         P->axis_tw[0].table = roots;
+    } else if (mode != MP_VALU) {
+        // complex64 on the matrix cores where a kernel covers the axis
+        rc = pl.axis_mp(nx, Geo{1, 0, ny * nz, nx, 1}, &cur, 0, mode);
+        if (!rc) rc = pl.axis_mp(ny, Geo{nz, nx * ny, nx, 1, nx}, &cur, 1, mode);
+        if (!rc && shape >= 3) rc = pl.axis_mp(nz, Geo{1, 0, nx * ny, 1, nx * ny}, &cur, 2, mode);
     } else if (shape == 2 && nx <= kMaxFused && ny <= kMaxFused && ny >= 4 && ny >= dif2_min_rows()) {
         rc = pl.plan_2d_dif2(nx, ny, &cur);
     } else {
@@ -1034,8 +1124,8 @@ void tope3DDestroy(struct topeFFT *f, struct topePlan3D *t)
 
 /* ---- 2D/3D plans in a chosen precision ------------------------------------ */
 
-void tope2DPlanInitEx(struct topeFFT *f, struct topePlan2D *t, int x, int y, int type, int precision,
-                      const void *host)
+void tope2DPlanInitMode(struct topeFFT *f, struct topePlan2D *t, int x, int y, int type, int precision, int mode,
+                        const void *host)
 {
     if (!f || !t) return;
     std::memset(t, 0, sizeof(*t));
@@ -1046,7 +1136,7 @@ void tope2DPlanInitEx(struct topeFFT *f, struct topePlan2D *t, int x, int y, int
     ref_radix(x, &t->radX, &t->logX);
     ref_radix(y, &t->radY, &t->logY);
     PlanImpl *P = nullptr;
-    int rc = build_plan(f, 2, x, y, 1, precision == TOPE_PREC_F32 ? PREC_F32 : PREC_F64, MP_VALU, host, &P,
+    int rc = build_plan(f, 2, x, y, 1, precision == TOPE_PREC_F32 ? PREC_F32 : PREC_F64, mode, host, &P,
                         &t->totalPreKernel, &t->totalMemory);
     if (!rc && precision != TOPE_PREC_F64 && precision != TOPE_PREC_F32) rc = TOPE_ERROR_INVALID_ARG;
     f->error = rc;
@@ -1062,8 +1152,14 @@ void tope2DPlanInitEx(struct topeFFT *f, struct topePlan2D *t, int x, int y, int
     sync_fields(t, P);
 }
 
-void tope3DPlanInitEx(struct topeFFT *f, struct topePlan3D *t, int x, int y, int z, int type, int precision,
+void tope2DPlanInitEx(struct topeFFT *f, struct topePlan2D *t, int x, int y, int type, int precision,
                       const void *host)
+{
+    tope2DPlanInitMode(f, t, x, y, type, precision, TOPE_MODE_VALU, host);
+}
+
+void tope3DPlanInitMode(struct topeFFT *f, struct topePlan3D *t, int x, int y, int z, int type, int precision,
+                        int mode, const void *host)
 {
     if (!f || !t) return;
     std::memset(t, 0, sizeof(*t));
@@ -1076,7 +1172,7 @@ void tope3DPlanInitEx(struct topeFFT *f, struct topePlan3D *t, int x, int y, int
     ref_radix(y, &t->radY, &t->logY);
     ref_radix(z, &t->radZ, &t->logZ);
     PlanImpl *P = nullptr;
-    int rc = build_plan(f, 3, x, y, z, precision == TOPE_PREC_F32 ? PREC_F32 : PREC_F64, MP_VALU, host, &P,
+    int rc = build_plan(f, 3, x, y, z, precision == TOPE_PREC_F32 ? PREC_F32 : PREC_F64, mode, host, &P,
                         &t->totalPreKernel, &t->totalMemory);
     if (!rc && precision != TOPE_PREC_F64 && precision != TOPE_PREC_F32) rc = TOPE_ERROR_INVALID_ARG;
     f->error = rc;
@@ -1091,6 +1187,12 @@ void tope3DPlanInitEx(struct topeFFT *f, struct topePlan3D *t, int x, int y, int
     t->twdY = P->axis_tw[1].table;
     t->twdZ = P->axis_tw[2].table;
     sync_fields(t, P);
+}
+
+void tope3DPlanInitEx(struct topeFFT *f, struct topePlan3D *t, int x, int y, int z, int type, int precision,
+                      const void *host)
+{
+    tope3DPlanInitMode(f, t, x, y, z, type, precision, TOPE_MODE_VALU, host);
 }
 
 void tope2DExecEx(struct topeFFT *f, struct topePlan2D *t, void *host, int dir)

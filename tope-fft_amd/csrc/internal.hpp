@@ -64,6 +64,11 @@ hipError_t gen_mfma_constants(void *out, hipStream_t s);
 size_t mfma_constants_bytes();
 bool mfma_supported(int n);
 hipError_t launch_mfma_rows(int mode, int n, const void *in, void *out, const MfmaArgs &a, int sign, hipStream_t s);
+// Mixed-precision column tiles (mfma_tile.hpp): 16 unit-stride lines per block.
+struct MfmaTileArgs;
+bool mfma_tile_supported(int n);
+hipError_t launch_mfma_tile(int mode, int n, const void *in, void *out, const MfmaTileArgs &a, unsigned grid, int sign,
+                            hipStream_t s);
 
 // Framework plumbing (api.cpp) for modules outside it.
 hipStream_t fw_stream(struct topeFFT *f);

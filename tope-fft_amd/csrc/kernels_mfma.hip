@@ -4,6 +4,7 @@
 
 #include "internal.hpp"
 #include "mfma_kernel.hpp"
+#include "mfma_tile.hpp"
 
 namespace tope {
 
@@ -127,6 +128,44 @@ static hipError_t launch_mode(int n, const void *in, void *out, const MfmaArgs &
 }
 
 bool mfma_supported(int n) { return n == 256 || n == 4096; }
+
+template <int MODE, int N>
+static hipError_t launch_tile_n(const void *in, void *out, const MfmaTileArgs &a, unsigned grid, int sign,
+                                hipStream_t s)
+{
+    const cpx<float> *i = static_cast<const cpx<float> *>(in);
+    cpx<float> *o = static_cast<cpx<float> *>(out);
+    constexpr int T = MfmaTile<N>::THREADS;
+    if (sign < 0) hipLaunchKernelGGL((mfma_tile_kernel<MODE, N, -1>), dim3(grid), dim3(T), 0, s, i, o, a);
+    else hipLaunchKernelGGL((mfma_tile_kernel<MODE, N, 1>), dim3(grid), dim3(T), 0, s, i, o, a);
+    return hipGetLastError();
+}
+
+template <int MODE>
+static hipError_t launch_tile_mode(int n, const void *in, void *out, const MfmaTileArgs &a, unsigned grid, int sign,
+                                   hipStream_t s)
+{
+    switch (n) {
+    case 256: return launch_tile_n<MODE, 256>(in, out, a, grid, sign, s);
+    case 512: return launch_tile_n<MODE, 512>(in, out, a, grid, sign, s);
+    case 1024: return launch_tile_n<MODE, 1024>(in, out, a, grid, sign, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+bool mfma_tile_supported(int n) { return n == 256 || n == 512 || n == 1024; }
+
+hipError_t launch_mfma_tile(int mode, int n, const void *in, void *out, const MfmaTileArgs &a, unsigned grid, int sign,
+                            hipStream_t s)
+{
+    switch (mode) {
+    case MP_BF16X3: return launch_tile_mode<MP_BF16X3>(n, in, out, a, grid, sign, s);
+    case MP_BF16X6: return launch_tile_mode<MP_BF16X6>(n, in, out, a, grid, sign, s);
+    case MP_FP32: return launch_tile_mode<MP_FP32>(n, in, out, a, grid, sign, s);
+    case MP_FP16X3: return launch_tile_mode<MP_FP16X3>(n, in, out, a, grid, sign, s);
+    default: return hipErrorInvalidValue;
+    }
+}
 
 hipError_t launch_mfma_rows(int mode, int n, const void *in, void *out, const MfmaArgs &a, int sign, hipStream_t s)
 {

@@ -111,6 +111,8 @@ SIGNATURES = {
     "topeBatchDestroy": (None, [_F, _PB]),
     "tope2DPlanInitEx": (None, [_F, _P2, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]),
     "tope3DPlanInitEx": (None, [_F, _P3, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]),
+    "tope2DPlanInitMode": (None, [_F, _P2, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]),
+    "tope3DPlanInitMode": (None, [_F, _P3, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]),
     "tope2DExecEx": (None, [_F, _P2, C.c_void_p, C.c_int]),
     "tope3DExecEx": (None, [_F, _P3, C.c_void_p, C.c_int]),
     "topeManyPlanInit": (None, [_F, _PM, C.c_int, C.POINTER(C.c_int), C.c_int, C.c_void_p]),
@@ -191,7 +193,7 @@ class Plan:
     or flat; PlanInit uploads it, exec() transforms and downloads like the
     reference, exec_device() transforms on the device only."""
 
-    def __init__(self, fw, dims, x, type_=C2C, precision=None):
+    def __init__(self, fw, dims, x, type_=C2C, precision=None, mode=None):
         self.fw, self.lib = fw, fw.lib
         self.dims = tuple(int(d) for d in dims)
         self.ndim = len(self.dims)
@@ -204,17 +206,26 @@ class Plan:
         self.n = n
         self.host = host.view(np.float32 if precision == PREC_F32 else np.float64)
         if precision is not None:
+            hp = host.ctypes.data_as(C.c_void_p)
             if self.ndim == 2:
                 self.t = TopePlan2D()
-                self.lib.tope2DPlanInitEx(C.byref(fw.f), C.byref(self.t), self.dims[0], self.dims[1], type_,
-                                          precision, host.ctypes.data_as(C.c_void_p))
+                if mode is None:
+                    self.lib.tope2DPlanInitEx(C.byref(fw.f), C.byref(self.t), self.dims[0], self.dims[1], type_,
+                                              precision, hp)
+                else:
+                    self.lib.tope2DPlanInitMode(C.byref(fw.f), C.byref(self.t), self.dims[0], self.dims[1], type_,
+                                                precision, mode, hp)
             elif self.ndim == 3:
                 self.t = TopePlan3D()
-                self.lib.tope3DPlanInitEx(C.byref(fw.f), C.byref(self.t), self.dims[0], self.dims[1],
-                                          self.dims[2], type_, precision, host.ctypes.data_as(C.c_void_p))
+                if mode is None:
+                    self.lib.tope3DPlanInitEx(C.byref(fw.f), C.byref(self.t), self.dims[0], self.dims[1],
+                                              self.dims[2], type_, precision, hp)
+                else:
+                    self.lib.tope3DPlanInitMode(C.byref(fw.f), C.byref(self.t), self.dims[0], self.dims[1],
+                                                self.dims[2], type_, precision, mode, hp)
             else:
                 raise ValueError("precision plans are 2D or 3D (1D: BatchPlan)")
-            fw.check(f"tope{self.ndim}DPlanInitEx{self.dims}")
+            fw.check(f"tope{self.ndim}DPlanInit{'Ex' if mode is None else 'Mode'}{self.dims}")
             return
         if self.ndim == 1:
             self.t = TopePlan1D()
