@@ -157,7 +157,9 @@ def uneven_batch_leg(fw, steps, warmup):
 def large_3d_leg(fw, steps, warmup):
     """3D 512^3: Gaussian random field with |k|^-2 power spectrum (k-space
     synthesis, inverse transform by the FP64 plan) plus 1e-3 white noise;
-    FP64 (2 GiB) and complex64 (1 GiB) plans, forward then inverse."""
+    FP64 (2 GiB) and complex64 (1 GiB) plans -- FP32 VALU axis passes, and
+    split-fp16 MFMA passes where a kernel covers the axis -- forward then
+    inverse."""
     import topefft
     import workloads
     dims = (512, 512, 512)
@@ -170,9 +172,10 @@ def large_3d_leg(fw, steps, warmup):
     field += 1e-3 * workloads.complex_gaussian(workloads.SEED_3D_LARGE + 1, n)
     res = {"workload": "3D C2C 512^3, |k|^-2 Gaussian random field + 1e-3 white noise, forward then inverse"}
     ref = None
-    for prec, name, sb in ((None, "fp64", 16), (topefft.PREC_F32, "complex64", 8)):
+    for prec, mode, name, sb in ((None, None, "fp64", 16), (topefft.PREC_F32, None, "complex64", 8),
+                                 (topefft.PREC_F32, topefft.MODE_MFMA_FP16X3, "complex64_mfma_fp16x3", 8)):
         xs = field if prec is None else field.astype(np.complex64)
-        q = topefft.Plan(fw, dims, xs, precision=prec)
+        q = topefft.Plan(fw, dims, xs, precision=prec, mode=mode)
         X = q.exec(topefft.FORWARD)
         back = q.exec(topefft.INVERSE)
         t, pass_ms = measure(fw, q, steps, warmup)
@@ -187,6 +190,7 @@ def large_3d_leg(fw, steps, warmup):
             ref = Xd
         else:
             entry["rel_l2_vs_fp64"] = float(f"{np.linalg.norm(Xd - ref) / np.linalg.norm(ref):.3e}")
+            entry["max_abs_vs_fp64"] = float(f"{np.max(np.abs(Xd - ref)):.3e}")
         del X, back, Xd, xs
         res[name] = entry
     return res

@@ -33,12 +33,16 @@ namespace tope {
 // i's bits above bit 1, so that every access of the kernel -- lanes over 16
 // lines and four rows whose indices differ in one 2-bit field of i -- hits
 // 64 distinct banks (and each half wave 32).  Linear over GF(2) in (i, c).
-__device__ __forceinline__ int tile_lds(int i, int c)
+__device__ __forceinline__ constexpr int tile_lds(int i, int c)
 {
     const int k0 = __builtin_popcount(i >> 2) & 1;
     const int k1 = __builtin_popcount((i >> 3) & 0x55555555) & 1;
     return ((i << 4) | c) ^ ((k0 | (k1 << 1)) << 4);
 }
+// Every element index the kernel forms is lane + k with lane and k in
+// disjoint bits (lane: wave, g; k: compile-time q, r, s, u, m), so its LDS
+// word is tile_lds(lane, c) ^ tile_lds(k, 0) and its byte offset in the
+// line is (c + lane is) 8 (per lane) + k is 8 (uniform).
 
 struct MfmaTileArgs {
     long long O, os;       // outer index o < O at element offset o * os
@@ -81,8 +85,15 @@ __global__ void __launch_bounds__(MfmaTile<N>::THREADS) mfma_tile_kernel(const c
     const long long blk = xcd_block(a.xcd);
     const long long o = blk / a.tiles;
     const long long first = o * a.os + (blk % a.tiles) * 16;  // element offset of line 0 of the tile
-    // byte offset of element i of this lane's line (fits 31 bits: checked on the host)
-    auto voff = [&](int i) { return (int)((c + (long long)i * a.is) * CB); };
+    // byte offsets fit 31 bits (checked on the host): per-lane part + uniform part
+    const int is8 = (int)(a.is * CB);
+    auto voff = [&](int lane_i) { return (c + lane_i * (int)a.is) * CB; };
+
+    // lane parts of the element indices
+    const int li_mfma = wave * GPW + NB * 4 * g;  // MFMA stage inputs and the final outputs
+    const int lw_mfma = tile_lds(li_mfma, c) * 4;  // LDS byte offsets (XOR distributes over * 4)
+    const int v_mfma = voff(li_mfma);
+    auto word = [&](int byte_off) -> float & { return *reinterpret_cast<float *>(reinterpret_cast<char *>(lds) + byte_off); };
 
     float x[GPW][8];  // slots 0..3: real parts, 4..7: imaginary parts
     {
@@ -90,12 +101,13 @@ __global__ void __launch_bounds__(MfmaTile<N>::THREADS) mfma_tile_kernel(const c
         if constexpr (R > 1) {
             // stage-0 butterflies j = wave + W g + NB u (u < 16/R), inputs j + (N/R) m;
             // element e = u R + m sits in x[e / 4][e % 4] (re) and x[e / 4][4 + e % 4] (im)
+            const int v0 = voff(wave + W * g);
 #pragma unroll
             for (int u = 0; u < 16 / R; ++u)
 #pragma unroll
                 for (int m = 0; m < R; ++m) {
-                    const int e = u * R + m, j = wave + W * g + NB * u;
-                    const cpx<float> v = buf_load<float>(rin, voff(j + (N / R) * m), 0);
+                    const int e = u * R + m;
+                    const cpx<float> v = buf_load<float>(rin, v0, (NB * u + (N / R) * m) * is8);
                     x[e / 4][e % 4] = v.x;
                     x[e / 4][4 + e % 4] = v.y;
                 }
@@ -104,8 +116,7 @@ __global__ void __launch_bounds__(MfmaTile<N>::THREADS) mfma_tile_kernel(const c
             for (int q = 0; q < GPW; ++q)
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
-                    const int j = wave * GPW + q;
-                    const cpx<float> v = buf_load<float>(rin, voff(j + NB * (4 * g + s)), 0);
+                    const cpx<float> v = buf_load<float>(rin, v_mfma, (q + NB * s) * is8);
                     x[q][s] = v.x;
                     x[q][4 + s] = v.y;
                 }
@@ -157,6 +168,7 @@ __global__ void __launch_bounds__(MfmaTile<N>::THREADS) mfma_tile_kernel(const c
                 x[e / 4][4 + e % 4] = v[m].y;
             }
         }
+        const int lw0 = tile_lds((wave + W * g) * R, c) * 4;
 #pragma unroll
         for (int part = 0; part < 2; ++part) {
             if (part > 0) __syncthreads();
@@ -164,15 +176,14 @@ __global__ void __launch_bounds__(MfmaTile<N>::THREADS) mfma_tile_kernel(const c
             for (int u = 0; u < 16 / R; ++u)
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    const int e = u * R + r, j = wave + W * g + NB * u;
-                    lds[tile_lds(j * R + r, c)] = x[e / 4][4 * part + e % 4];
+                    const int e = u * R + r;
+                    word(lw0 ^ (tile_lds(NB * R * u + r, 0) * 4)) = x[e / 4][4 * part + e % 4];
                 }
             __syncthreads();
 #pragma unroll
             for (int q = 0; q < GPW; ++q)
 #pragma unroll
-                for (int s = 0; s < 4; ++s)
-                    x[q][4 * part + s] = lds[tile_lds(wave * GPW + q + NB * (4 * g + s), c)];
+                for (int s = 0; s < 4; ++s) x[q][4 * part + s] = word(lw_mfma ^ (tile_lds(q + NB * s, 0) * 4));
         }
     }
 
@@ -210,25 +221,27 @@ __global__ void __launch_bounds__(MfmaTile<N>::THREADS) mfma_tile_kernel(const c
                 for (int r = 0; r < 4; ++r) {
                     cpx<float> v{x[q][r], x[q][4 + r]};
                     if constexpr (SCALED) v = scale(v, out_scale);
-                    buf_store<float>(v, rout, voff(wave * GPW + q + NB * (4 * g + r)), 0);
+                    buf_store<float>(v, rout, v_mfma, (q + NB * r) * is8);
                 }
         } else {
+            // output r of butterfly j = wave GPW + q: (j / Ns) Ns 16 + j % Ns + (4g + r) Ns
+            const int lNs = ilog2c(Ns);
+            const int jl = wave * GPW;
+            const int lw = tile_lds(((jl >> lNs) << lNs) * 16 + (jl & (Ns - 1)) + 4 * g * Ns, c) * 4;
 #pragma unroll
             for (int part = 0; part < 2; ++part) {
                 __syncthreads();  // the previous reads of lds are done
 #pragma unroll
-                for (int q = 0; q < GPW; ++q) {
-                    const int j = wave * GPW + q;
-                    const int base = (j / Ns) * Ns * 16 + (j & (Ns - 1));
+                for (int q = 0; q < GPW; ++q)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) lds[tile_lds(base + (4 * g + r) * Ns, c)] = x[q][4 * part + r];
-                }
+                    for (int r = 0; r < 4; ++r)
+                        word(lw ^ (tile_lds(((q >> lNs) << lNs) * 16 + (q & (Ns - 1)) + r * Ns, 0) * 4)) =
+                            x[q][4 * part + r];
                 __syncthreads();
 #pragma unroll
                 for (int q = 0; q < GPW; ++q)
 #pragma unroll
-                    for (int s = 0; s < 4; ++s)
-                        x[q][4 * part + s] = lds[tile_lds(wave * GPW + q + NB * (4 * g + s), c)];
+                    for (int s = 0; s < 4; ++s) x[q][4 * part + s] = word(lw_mfma ^ (tile_lds(q + NB * s, 0) * 4));
             }
         }
     }
