@@ -116,7 +116,7 @@ def mixed_precision_leg(fw, steps, warmup):
     return res
 
 
-def uneven_batch_leg(fw, steps, warmup):
+def uneven_batch_leg(fw, steps, warmup, graphs=True):
     """20,000 transforms, lengths 2^6..2^14 log-uniform, packed with an
     offset table: one persistent launch (direct and hipGraph replay, after
     autotuning) against one launch per transform (direct and graph)."""
@@ -133,8 +133,9 @@ def uneven_batch_leg(fw, steps, warmup):
         p.tune(None, reps=3)
         radix, grid = p.schedule()
         entry = {"schedule_radix_by_log2n": {str(L): radix[L] for L in range(6, 15)}, "grid": grid}
-        for m in (topefft.MANY_PERSISTENT, topefft.MANY_PERSISTENT_GRAPH, topefft.MANY_PER_TRANSFORM,
-                  topefft.MANY_PER_TRANSFORM_GRAPH):
+        methods = (topefft.MANY_PERSISTENT, topefft.MANY_PERSISTENT_GRAPH, topefft.MANY_PER_TRANSFORM,
+                   topefft.MANY_PER_TRANSFORM_GRAPH)
+        for m in methods if graphs else methods[::2]:
             p.set_method(m)
             k = steps if m in (topefft.MANY_PERSISTENT, topefft.MANY_PERSISTENT_GRAPH) else 3
             dirs = [topefft.FORWARD, topefft.INVERSE]
@@ -234,6 +235,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="uneven batch: skip the hipGraph-replay methods (for kernel-trace profiling)")
     ap.add_argument("--legs", default="all",
                     help="comma list of 2d,3d256,mp,many,3d512 (default all; 2d is always run)")
     args = ap.parse_args()
@@ -281,7 +284,7 @@ def main():
     if "mp" in legs:
         extra["mixed_precision_batch"] = mixed_precision_leg(fw, max(5, args.steps // 2), args.warmup)
     if "many" in legs:
-        extra["uneven_batch"] = uneven_batch_leg(fw, max(5, args.steps // 2), args.warmup)
+        extra["uneven_batch"] = uneven_batch_leg(fw, max(5, args.steps // 2), args.warmup, not args.no_graph)
     if "3d512" in legs:
         extra["3d_512cubed"] = large_3d_leg(fw, 4, 1)
 
