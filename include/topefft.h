@@ -246,6 +246,30 @@ void tope3DPlanInitMode(struct topeFFT *f, struct topePlan3D *t, int x, int y, i
 void tope2DExecEx(struct topeFFT *f, struct topePlan2D *t, void *host, int dir);
 void tope3DExecEx(struct topeFFT *f, struct topePlan3D *t, void *host, int dir);
 
+/* ---- real-input transforms (no reference counterpart; the reference
+ * declares R2C but runs C2C) ---------------------------------------------
+ * x (even, x/2 <= 4096) by y by z (<= 4096 each; 1 for unused axes), powers
+ * of two, FP64, x fastest.  PlanInit uploads the real array (x*y*z doubles).
+ * topeRealExec FORWARD (R2C) transforms the device real array into the half
+ * spectrum, (x/2+1)*y*z complex128 in the layout of numpy's rfftn (kx
+ * fastest), and downloads it into host (2*(x/2+1)*y*z doubles); INVERSE
+ * (C2R) transforms the device half spectrum back into the real array,
+ * scaled by 1/(x*y*z), and downloads x*y*z doubles.  FORWARD consumes the
+ * real array and INVERSE the half spectrum (in-place pipeline). */
+struct topePlanReal {
+    int x, y, z, dim;
+    void *data;      /* device half spectrum */
+    void *real;      /* device real array */
+    size_t dataSize; /* bytes of data */
+    size_t realSize; /* bytes of real */
+    tope_ulong totalMemory, totalPreKernel, totalKernel;
+    void *impl;
+};
+void topeRealPlanInit(struct topeFFT *f, struct topePlanReal *t, int x, int y, int z, const double *real);
+void topeRealExec(struct topeFFT *f, struct topePlanReal *t, double *host, int dir);
+void topeRealExecDevice(struct topeFFT *f, struct topePlanReal *t, int dir);
+void topeRealDestroy(struct topeFFT *f, struct topePlanReal *t);
+
 /* ---- packed batch of uneven power-of-two lengths (no reference
  * counterpart) ---------------------------------------------------------
  * `count` transforms with lengths sizes[i] (powers of two, 16 .. 16384)
@@ -291,7 +315,7 @@ void topeManyDestroy(struct topeFFT *f, struct topePlanMany *t);
 
 /* Size of the public structs, so foreign-language bindings can check their
  * mirrors of this header. which: 0 topeFFT, 1 topePlan1D, 2 topePlan2D,
- * 3 topePlan3D, 4 topePlanBatch, 5 topePlanMany. */
+ * 3 topePlan3D, 4 topePlanBatch, 5 topePlanMany, 6 topePlanReal. */
 size_t topeStructSize(int which);
 
 /* Human-readable message for a status code. */

@@ -1,0 +1,95 @@
+"""Real-input transforms (topeReal*): R2C half spectrum against the FP64
+oracle's full complex transform (sliced to kx <= nx/2), C2R by round trip and
+against the oracle's inverse, closed forms, and the size limits.  Bar:
+rel-L2 <= 1e-12, as for the complex FP64 path."""
+import numpy as np
+import pytest
+
+import topefft
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(2,), (4,), (16,), (256,), (4096,), (8192,), (8, 4), (64, 32), (256, 256), (2, 8),
+          (16, 8, 4), (32, 32, 32), (2, 4, 8), (128, 2, 16)]
+
+
+def real_input(dims, seed):
+    rng = np.random.default_rng(seed)
+    return rng.uniform(-1.0, 1.0, int(np.prod(dims)))
+
+
+def half(spectrum, dims):
+    nx = dims[0]
+    ny = dims[1] if len(dims) > 1 else 1
+    nz = dims[2] if len(dims) > 2 else 1
+    return spectrum.reshape(nz, ny, nx)[:, :, : nx // 2 + 1].reshape(-1)
+
+
+@pytest.mark.parametrize("dims", SHAPES, ids=lambda d: "x".join(map(str, d)))
+def test_r2c_c2r(oracle, dims):
+    x = real_input(dims, 40 + int(np.prod(dims)) % 31)
+    fw = topefft.Framework()
+    p = topefft.RealPlan(fw, dims, x)
+    X = p.forward()
+    back = p.inverse()
+    names = p.pass_names()
+    p.destroy()
+    fw.destroy()
+    full = oracle.dft(x.astype(np.complex128), dims, topefft.FORWARD)
+    want = half(full, dims)
+    assert X.size == want.size
+    assert rel_l2(X, want) < 1e-12, names
+    assert rel_l2(back, x) < 1e-12
+
+
+def test_c2r_against_oracle_inverse(oracle):
+    # the C2R output equals the oracle's inverse of the full Hermitian spectrum
+    dims = (64, 16, 8)
+    x = real_input(dims, 3)
+    fw = topefft.Framework()
+    p = topefft.RealPlan(fw, dims, x)
+    p.exec_device(topefft.FORWARD)
+    got = p.inverse()
+    p.destroy()
+    fw.destroy()
+    full = oracle.dft(x.astype(np.complex128), dims, topefft.FORWARD)
+    want = oracle.dft(full, dims, topefft.INVERSE)
+    assert np.max(np.abs(want.imag)) < 1e-12
+    assert rel_l2(got, want.real) < 1e-12
+
+
+def r2c(dims, x):
+    # a fresh framework per plan: destroying the last plan releases it
+    fw = topefft.Framework()
+    p = topefft.RealPlan(fw, dims, x)
+    try:
+        return p.forward()
+    finally:
+        p.destroy()
+        fw.destroy()
+
+
+def test_r2c_closed_forms():
+    n = 1024
+    # delta at 0 -> all ones
+    d = np.zeros(n)
+    d[0] = 1.0
+    assert np.allclose(r2c((n,), d), 1.0, atol=1e-14)
+    # cos(2 pi k0 j / n) -> n/2 at k0, zero elsewhere
+    k0 = 37
+    c = np.cos(2 * np.pi * k0 * np.arange(n) / n)
+    want = np.zeros(n // 2 + 1, complex)
+    want[k0] = n / 2
+    assert np.max(np.abs(r2c((n,), c) - want)) < 1e-11
+
+
+def test_real_plan_errors():
+    fw = topefft.Framework()
+    with pytest.raises(topefft.TopeError) as e:
+        topefft.RealPlan(fw, (6,), np.zeros(6))
+    assert e.value.code == topefft.TOPE_ERROR_INVALID_SIZE
+    with pytest.raises(topefft.TopeError) as e:
+        topefft.RealPlan(fw, (16384,), np.zeros(16384))
+    assert e.value.code == topefft.TOPE_ERROR_UNSUPPORTED
+    fw.destroy()

@@ -44,7 +44,7 @@ struct FrameworkImpl {
     EventPair last;  // last exec's device time (profileThis)
 };
 
-enum PassKind { PASS_AXIS = 0, PASS_DFT = 1, PASS_MFMA = 2, PASS_MFMA_TILE = 3 };
+enum PassKind { PASS_AXIS = 0, PASS_DFT = 1, PASS_MFMA = 2, PASS_MFMA_TILE = 3, PASS_REAL = 4 };
 
 struct Pass {
     int kind = PASS_AXIS;
@@ -60,6 +60,7 @@ struct Pass {
     bool last = false;                            // receives the inverse 1/N scale
     long long dft_n = 0;
     const void *dft_roots = nullptr;
+    const void *real_w = nullptr;  // PASS_REAL: W_n^k, k = 0..m (FP64)
     const char *name = "";
 };
 
@@ -81,6 +82,13 @@ struct PlanImpl {
         int n = 0, e = 0;
         void *table = nullptr;
     } axis_tw[3];
+    // real-input plans: passes run in inv_order on INVERSE (C2R), the
+    // half-spectrum rows have m + 1 elements, the real array N doubles
+    bool real = false;
+    long long real_m = 0, real_rows = 0;
+    double real_scale = 1.0;
+    size_t real_bytes = 0;
+    std::vector<int> inv_order;
     EventPair run;  // this plan's exec events
     // per-pass timing
     bool timing = false;
@@ -621,6 +629,12 @@ int launch_pass(PlanImpl *P, size_t i, hipStream_t s, int dir)
         m.scale = ps.last ? (float)sc : 1.0f;  // 1/N is a power of two: exact in FP32
         hipError_t err = launch_mfma_rows(ps.mfma_mode, ps.mfma_n, ib, ob, m, sign, s);
         if (err != hipSuccess) return map_err(err);
+    } else if (ps.kind == PASS_REAL) {
+        const bool fwd = dir == FORWARD;
+        const void *src = P->buf[fwd ? ps.in_buf : ps.out_buf];
+        void *dst = P->buf[fwd ? ps.out_buf : ps.in_buf];
+        hipError_t err = launch_real_pass(fwd ? 0 : 1, src, dst, ps.real_w, P->real_m, P->real_rows, P->real_scale, s);
+        if (err != hipSuccess) return map_err(err);
     } else if (ps.kind == PASS_MFMA_TILE) {
         MfmaTileArgs m = ps.mt;
         m.scale = ps.last ? (float)sc : 1.0f;
@@ -637,7 +651,9 @@ int launch_pass(PlanImpl *P, size_t i, hipStream_t s, int dir)
 int enqueue(PlanImpl *P, hipStream_t s, int dir)
 {
     if (dir != FORWARD && dir != INVERSE) return TOPE_ERROR_INVALID_ARG;
-    for (size_t i = 0; i < P->passes.size(); ++i) {
+    const bool rev = dir == INVERSE && !P->inv_order.empty();
+    for (size_t n = 0; n < P->passes.size(); ++n) {
+        const size_t i = rev ? (size_t)P->inv_order[n] : n;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (P->timing) {
             e0 = pool_event(P);
@@ -809,6 +825,7 @@ int build_plan(struct topeFFT *f, int shape, long long nx, long long ny, long lo
     return TOPE_SUCCESS;
 }
 
+
 void free_plan(PlanImpl *P)
 {
     if (!P) return;
@@ -821,6 +838,94 @@ void free_plan(PlanImpl *P)
     if (P->run.stop) (void)hipEventDestroy(P->run.stop);
     delete P;
 }
+
+// Real-input plan: x (even, x/2 <= 4096) by y by z (y, z <= 4096), all powers
+// of two.  buf[0] holds the real array (x y z doubles, read as (x/2) y z
+// complex), buf[1] the half spectrum ((x/2 + 1) y z complex).  FORWARD (R2C):
+// m-point rows in buf[0], post-processing into buf[1], then Y and Z in
+// buf[1]; INVERSE (C2R) runs the same passes in reverse order, the
+// pre-processing scaled by 1/(m y z) so the result carries 1/(x y z).
+int build_real_plan(struct topeFFT *f, long long nx, long long ny, long long nz, const double *real, PlanImpl **out,
+                    tope_ulong *pre_ns, tope_ulong *mem_ns)
+{
+    FrameworkImpl *fw = fw_of(f);
+    if (!fw) return TOPE_ERROR_NOT_INITIALIZED;
+    if (!real) return TOPE_ERROR_INVALID_ARG;
+    if (nx < 2 || ny < 1 || nz < 1 || !is_pow2(nx) || !is_pow2(ny) || !is_pow2(nz)) return TOPE_ERROR_INVALID_SIZE;
+    if (nx / 2 > kMaxFused || ny > kMaxFused || nz > kMaxFused) return TOPE_ERROR_UNSUPPORTED;
+    const long long m = nx / 2, rows = ny * nz;
+    if ((m + 1) * rows > (1LL << 33)) return TOPE_ERROR_INVALID_SIZE;
+
+    PlanImpl *P = new (std::nothrow) PlanImpl();
+    if (!P) return TOPE_ERROR_UNSUPPORTED;
+    P->fw = fw;
+    P->prec = PREC_F64;
+    P->elem = 16;
+    P->ndim = nz > 1 ? 3 : (ny > 1 ? 2 : 1);
+    P->dims[0] = nx;
+    P->dims[1] = ny;
+    P->dims[2] = nz;
+    P->total = (m + 1) * rows;  // elements of the larger (spectrum) buffer
+    P->norm = 1;                // axis passes unscaled; the C2R pre-pass scales
+    P->bytes = (size_t)P->total * P->elem;
+    P->real = true;
+    P->real_m = m;
+    P->real_rows = rows;
+    P->real_scale = 1.0 / ((double)m * (double)rows);
+    P->real_bytes = (size_t)(nx * rows) * sizeof(double);
+    hipStream_t s = fw->stream;
+
+    auto fail = [&](int rc) {
+        free_plan(P);
+        return rc;
+    };
+    hipError_t err = hipMalloc(&P->buf[0], P->bytes);
+    if (err == hipSuccess) err = hipMalloc(&P->buf[1], P->bytes);
+    if (err != hipSuccess) return fail(map_err(err));
+
+    EventPair pre;
+    err = time_start(pre, s);
+    if (err != hipSuccess) return fail(map_err(err));
+    Planner pl(P, s);
+    int cur = 0;
+    // X: m-point complex rows of the real array (buffer 0, in place)
+    int rc = pl.axis(m, Geo{1, 0, rows, m, 1}, &cur, 0);
+    if (!rc && cur != 0) rc = TOPE_ERROR_UNSUPPORTED;
+    if (!rc) {
+        void *w = nullptr;
+        err = pl.power_table(nx, 1, m + 1, &w);
+        if (err != hipSuccess) return fail(map_err(err));
+        Pass ps;
+        ps.kind = PASS_REAL;
+        ps.real_w = w;
+        ps.in_buf = 0;
+        ps.out_buf = 1;
+        ps.name = "real_half_spectrum";
+        P->passes.push_back(ps);
+        cur = 1;
+        const long long mp = m + 1;
+        rc = pl.axis(ny, Geo{nz, mp * ny, mp, 1, mp}, &cur, 1);
+        if (!rc) rc = pl.axis(nz, Geo{1, 0, mp * ny, 1, mp * ny}, &cur, 2);
+        if (!rc && cur != 1) rc = TOPE_ERROR_UNSUPPORTED;
+    }
+    if (rc) return fail(rc);
+    for (long long i = (long long)P->passes.size() - 1; i >= 0; --i) P->inv_order.push_back((int)i);
+    P->final_buf = 0;
+    err = time_stop(pre, s, pre_ns);
+    if (err != hipSuccess) return fail(map_err(err));
+    EventPair up;
+    err = time_start(up, s);
+    if (err == hipSuccess) err = hipMemcpyAsync(P->buf[0], real, P->real_bytes, hipMemcpyHostToDevice, s);
+    if (err == hipSuccess) err = time_stop(up, s, mem_ns);
+    (void)hipEventDestroy(pre.start);
+    (void)hipEventDestroy(pre.stop);
+    (void)hipEventDestroy(up.start);
+    (void)hipEventDestroy(up.stop);
+    if (err != hipSuccess) return fail(map_err(err));
+    *out = P;
+    return TOPE_SUCCESS;
+}
+
 
 void release_framework(struct topeFFT *f)
 {
@@ -1384,6 +1489,79 @@ const char *topePlanPassName(void *plan_impl, int pass)
     return P->passes[pass].name;
 }
 
+/* ---- real-input transforms (no reference counterpart) ------------------- */
+
+void topeRealPlanInit(struct topeFFT *f, struct topePlanReal *t, int x, int y, int z, const double *real)
+{
+    if (!f || !t) return;
+    std::memset(t, 0, sizeof(*t));
+    t->x = x;
+    t->y = y;
+    t->z = z;
+    PlanImpl *P = nullptr;
+    const int rc = build_real_plan(f, x, y, z, real, &P, &t->totalPreKernel, &t->totalMemory);
+    f->error = rc;
+    if (rc) return;
+    P->fw->live_plans++;
+    t->impl = P;
+    t->dim = P->ndim;
+    t->real = P->buf[0];
+    t->data = P->buf[1];
+    t->realSize = P->real_bytes;
+    t->dataSize = P->bytes;
+}
+
+void topeRealExec(struct topeFFT *f, struct topePlanReal *t, double *host, int dir)
+{
+    if (!f || !t) return;
+    PlanImpl *P = impl_of(t->impl);
+    if (!P || !P->real) {
+        f->error = TOPE_ERROR_NOT_INITIALIZED;
+        return;
+    }
+    FrameworkImpl *fw = fw_of(f);
+    if (!fw || P->fw != fw) {
+        f->error = TOPE_ERROR_NOT_INITIALIZED;
+        return;
+    }
+    if (!host) {
+        f->error = TOPE_ERROR_INVALID_ARG;
+        return;
+    }
+    hipStream_t s = fw->stream;
+    hipError_t err = time_start(fw->last, s);
+    int rc = err == hipSuccess ? enqueue(P, s, dir) : map_err(err);
+    if (!rc) rc = map_err(time_stop(fw->last, s, &t->totalKernel));
+    if (!rc) {
+        EventPair dn;
+        err = time_start(dn, s);
+        if (err == hipSuccess)
+            err = hipMemcpyAsync(host, dir == FORWARD ? P->buf[1] : P->buf[0],
+                                 dir == FORWARD ? P->bytes : P->real_bytes, hipMemcpyDeviceToHost, s);
+        if (err == hipSuccess) err = time_stop(dn, s, &t->totalMemory);
+        (void)hipEventDestroy(dn.start);
+        (void)hipEventDestroy(dn.stop);
+        rc = map_err(err);
+    }
+    f->error = rc;
+}
+
+void topeRealExecDevice(struct topeFFT *f, struct topePlanReal *t, int dir)
+{
+    if (!f || !t) return;
+    PlanImpl *P = impl_of(t->impl);
+    f->error = (P && P->real) ? exec_device(f, P, dir) : TOPE_ERROR_NOT_INITIALIZED;
+}
+
+void topeRealDestroy(struct topeFFT *f, struct topePlanReal *t)
+{
+    if (!f || !t) return;
+    destroy_common(f, impl_of(t->impl));
+    t->impl = nullptr;
+    t->data = t->real = nullptr;
+    f->error = TOPE_SUCCESS;
+}
+
 size_t topeStructSize(int which)
 {
     switch (which) {
@@ -1393,6 +1571,7 @@ size_t topeStructSize(int which)
     case 3: return sizeof(struct topePlan3D);
     case 4: return sizeof(struct topePlanBatch);
     case 5: return sizeof(struct topePlanMany);
+    case 6: return sizeof(struct topePlanReal);
     default: return 0;
     }
 }

@@ -67,6 +67,13 @@ class TopePlanMany(C.Structure):
                 ("impl", C.c_void_p)]
 
 
+class TopePlanReal(C.Structure):
+    _fields_ = [("x", C.c_int), ("y", C.c_int), ("z", C.c_int), ("dim", C.c_int),
+                ("data", C.c_void_p), ("real", C.c_void_p), ("dataSize", C.c_size_t), ("realSize", C.c_size_t),
+                ("totalMemory", C.c_uint64), ("totalPreKernel", C.c_uint64), ("totalKernel", C.c_uint64),
+                ("impl", C.c_void_p)]
+
+
 MANY_PERSISTENT, MANY_PERSISTENT_GRAPH, MANY_PER_TRANSFORM, MANY_PER_TRANSFORM_GRAPH = 0, 1, 2, 3
 MANY_METHOD_NAMES = {0: "persistent", 1: "persistent_graph", 2: "per_transform", 3: "per_transform_graph"}
 
@@ -80,7 +87,12 @@ _F, _P1, _P2, _P3 = C.POINTER(TopeFFT), C.POINTER(TopePlan1D), C.POINTER(TopePla
 _DP = C.POINTER(C.c_double)
 _PB = C.POINTER(TopePlanBatch)
 _PM = C.POINTER(TopePlanMany)
+_PR = C.POINTER(TopePlanReal)
 SIGNATURES = {
+    "topeRealPlanInit": (None, [_F, _PR, C.c_int, C.c_int, C.c_int, _DP]),
+    "topeRealExec": (None, [_F, _PR, _DP, C.c_int]),
+    "topeRealExecDevice": (None, [_F, _PR, C.c_int]),
+    "topeRealDestroy": (None, [_F, _PR]),
     "topeFFTInit": (None, [_F]),
     "topeFFTDestroy": (None, [_F]),
     "tope1DPlanInit": (None, [_F, _P1, C.c_int, C.c_int, _DP]),
@@ -148,8 +160,8 @@ def load(path=LIB_PATH):
 def check_layout():
     lib = load()
     mine = [C.sizeof(TopeFFT), C.sizeof(TopePlan1D), C.sizeof(TopePlan2D), C.sizeof(TopePlan3D),
-            C.sizeof(TopePlanBatch), C.sizeof(TopePlanMany)]
-    theirs = [lib.topeStructSize(i) for i in range(6)]
+            C.sizeof(TopePlanBatch), C.sizeof(TopePlanMany), C.sizeof(TopePlanReal)]
+    theirs = [lib.topeStructSize(i) for i in range(7)]
     return mine, theirs
 
 
@@ -357,6 +369,56 @@ class BatchPlan:
     def destroy(self):
         if self.t.impl:
             self.lib.topeBatchDestroy(C.byref(self.fw.f), C.byref(self.t))
+
+
+class RealPlan:
+    """topeRealPlanInit / Exec / ExecDevice / Destroy: real array of dims
+    (nx[, ny[, nz]]), x fastest; forward() gives the half spectrum of
+    numpy's rfftn layout (nz, ny, nx//2 + 1), inverse() the real array."""
+
+    def __init__(self, fw, dims, x):
+        self.fw, self.lib = fw, fw.lib
+        d = tuple(int(v) for v in dims) + (1,) * (3 - len(dims))
+        self.dims = d
+        self.n = d[0] * d[1] * d[2]
+        self.half = (d[0] // 2 + 1) * d[1] * d[2]
+        host = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        if host.size != self.n:
+            raise ValueError("input size does not match dims")
+        self.t = TopePlanReal()
+        self.lib.topeRealPlanInit(C.byref(fw.f), C.byref(self.t), d[0], d[1], d[2], _dptr(host))
+        fw.check(f"topeRealPlanInit{d}")
+
+    def forward(self):
+        out = np.empty(self.half, np.complex128)
+        self.lib.topeRealExec(C.byref(self.fw.f), C.byref(self.t), _dptr(out.view(np.float64)), FORWARD)
+        self.fw.check("topeRealExec(FORWARD)")
+        return out
+
+    def inverse(self):
+        out = np.empty(self.n, np.float64)
+        self.lib.topeRealExec(C.byref(self.fw.f), C.byref(self.t), _dptr(out), INVERSE)
+        self.fw.check("topeRealExec(INVERSE)")
+        return out
+
+    def exec_device(self, direction=FORWARD):
+        self.lib.topeRealExecDevice(C.byref(self.fw.f), C.byref(self.t), direction)
+        self.fw.check("topeRealExecDevice")
+
+    def pass_count(self):
+        return self.lib.topePlanPassCount(self.t.impl)
+
+    def pass_names(self):
+        return [self.lib.topePlanPassName(self.t.impl, i).decode() for i in range(self.pass_count())]
+
+    def time_pass(self, i, direction=FORWARD, reps=10):
+        ms = self.lib.topePlanTimePass(C.byref(self.fw.f), self.t.impl, i, direction, reps)
+        self.fw.check("topePlanTimePass")
+        return ms
+
+    def destroy(self):
+        if self.t.impl:
+            self.lib.topeRealDestroy(C.byref(self.fw.f), C.byref(self.t))
 
 
 class ManyPlan:
