@@ -197,6 +197,24 @@ def large_3d_leg(fw, steps, warmup):
     return res
 
 
+def real_2d_leg(fw, steps, warmup):
+    """R2C/C2R 2D 4096^2 FP64 (seeded uniform real input): forward and
+    inverse alternate as in the complex legs; algorithmic bytes per
+    transform = 2 axes x (read + write of the (nx/2+1) x ny half spectrum)."""
+    import topefft
+    import workloads
+    nx = ny = 4096
+    x = workloads.uniform(workloads.SEED_2D, nx * ny)
+    p = topefft.RealPlan(fw, (nx, ny), x)
+    t, pass_ms = measure(fw, p, steps, warmup)
+    names = p.pass_names()
+    p.destroy()
+    alg = 2 * 2 * 16 * (nx // 2 + 1) * ny
+    return {"workload": "2D R2C/C2R FP64 4096x4096 real, device-resident", "ms": round(t * 1e3, 4),
+            "transforms_per_s": round(1.0 / t, 2), "gbs_algorithmic": round(gbs(alg, t), 1),
+            "passes": [{"kernel": nm, "ms": round(float(m), 4)} for nm, m in zip(names, pass_ms)]}
+
+
 def small_1d_leg(fw, steps, warmup):
     """1D 2^16 FP64 (latency-bound, two launches of the four-step schedule):
     device time per transform and transforms/s."""
@@ -238,7 +256,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="uneven batch: skip the hipGraph-replay methods (for kernel-trace profiling)")
     ap.add_argument("--legs", default="all",
-                    help="comma list of 2d,3d256,mp,many,3d512 (default all; 2d is always run)")
+                    help="comma list of 1d,3d256,mp,many,3d512,r2c (default all; 2d is always run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -263,7 +281,7 @@ def main():
     dom_bytes = 2 * 16 * n2  # one read + one write of the array per pass
     achieved = gbs(dom_bytes, pass2[dom] * 1e-3)
 
-    legs = set(args.legs.split(",")) if args.legs != "all" else {"1d", "3d256", "mp", "many", "3d512"}
+    legs = set(args.legs.split(",")) if args.legs != "all" else {"1d", "3d256", "mp", "many", "3d512", "r2c"}
     extra = {}
     if "1d" in legs:
         extra["1d_65536"] = small_1d_leg(fw, args.steps, args.warmup)
@@ -285,6 +303,8 @@ def main():
         extra["mixed_precision_batch"] = mixed_precision_leg(fw, max(5, args.steps // 2), args.warmup)
     if "many" in legs:
         extra["uneven_batch"] = uneven_batch_leg(fw, max(5, args.steps // 2), args.warmup, not args.no_graph)
+    if "r2c" in legs:
+        extra["r2c_2d_4096"] = real_2d_leg(fw, args.steps, args.warmup)
     if "3d512" in legs:
         extra["3d_512cubed"] = large_3d_leg(fw, 4, 1)
 

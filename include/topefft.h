@@ -252,12 +252,14 @@ void tope3DExecEx(struct topeFFT *f, struct topePlan3D *t, void *host, int dir);
  * of two, FP64, x fastest.  PlanInit uploads the real array (x*y*z doubles).
  * topeRealExec FORWARD (R2C) transforms the device real array into the half
  * spectrum, (x/2+1)*y*z complex128 in the layout of numpy's rfftn (kx
- * fastest), and downloads it into host (2*(x/2+1)*y*z doubles); INVERSE
+ * fastest), and downloads it into host (2*(x/2+1)*y*z doubles; on the
+ * device the rows of x/2+1 values sit at a pitch of `pitch` elements); INVERSE
  * (C2R) transforms the device half spectrum back into the real array,
  * scaled by 1/(x*y*z), and downloads x*y*z doubles.  FORWARD consumes the
  * real array and INVERSE the half spectrum (in-place pipeline). */
 struct topePlanReal {
     int x, y, z, dim;
+    int pitch, reserved; /* device half-spectrum row pitch (elements) */
     void *data;      /* device half spectrum */
     void *real;      /* device real array */
     size_t dataSize; /* bytes of data */

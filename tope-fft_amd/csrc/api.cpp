@@ -57,6 +57,7 @@ struct Pass {
     unsigned grid = 0;
     int in_buf = 0, out_buf = 0;
     long long reps = 1, rep_in = 0, rep_out = 0;  // repeated launches (rare geometry)
+    long long base = 0;                           // element offset of the pass's geometry in both buffers
     bool last = false;                            // receives the inverse 1/N scale
     long long dft_n = 0;
     const void *dft_roots = nullptr;
@@ -85,7 +86,7 @@ struct PlanImpl {
     // real-input plans: passes run in inv_order on INVERSE (C2R), the
     // half-spectrum rows have m + 1 elements, the real array N doubles
     bool real = false;
-    long long real_m = 0, real_rows = 0;
+    long long real_m = 0, real_rows = 0, real_pitch = 0;
     double real_scale = 1.0;
     size_t real_bytes = 0;
     std::vector<int> inv_order;
@@ -616,13 +617,19 @@ int launch_pass(PlanImpl *P, size_t i, hipStream_t s, int dir)
         if (ps.var->prec == PREC_F64) {
             AxisArgs<double> a = ps.a;
             a.scale = ps.last ? sc : 1.0;
+            a.in_elems -= ps.base;
+            a.out_elems -= ps.base;
             for (long long r = 0; r < ps.reps; ++r)
-                ps.var->fn(ib + r * ps.rep_in * eb, ob + r * ps.rep_out * eb, &a, ps.grid, s, sign);
+                ps.var->fn(ib + (ps.base + r * ps.rep_in) * eb, ob + (ps.base + r * ps.rep_out) * eb, &a, ps.grid, s,
+                           sign);
         } else {
             AxisArgs<float> a = ps.a32;
             a.scale = ps.last ? (float)sc : 1.0f;
+            a.in_elems -= ps.base;
+            a.out_elems -= ps.base;
             for (long long r = 0; r < ps.reps; ++r)
-                ps.var->fn(ib + r * ps.rep_in * eb, ob + r * ps.rep_out * eb, &a, ps.grid, s, sign);
+                ps.var->fn(ib + (ps.base + r * ps.rep_in) * eb, ob + (ps.base + r * ps.rep_out) * eb, &a, ps.grid, s,
+                           sign);
         }
     } else if (ps.kind == PASS_MFMA) {
         MfmaArgs m = ps.m;
@@ -633,7 +640,8 @@ int launch_pass(PlanImpl *P, size_t i, hipStream_t s, int dir)
         const bool fwd = dir == FORWARD;
         const void *src = P->buf[fwd ? ps.in_buf : ps.out_buf];
         void *dst = P->buf[fwd ? ps.out_buf : ps.in_buf];
-        hipError_t err = launch_real_pass(fwd ? 0 : 1, src, dst, ps.real_w, P->real_m, P->real_rows, P->real_scale, s);
+        hipError_t err =
+            launch_real_pass(fwd ? 0 : 1, src, dst, ps.real_w, P->real_m, P->real_pitch, P->real_rows, P->real_scale, s);
         if (err != hipSuccess) return map_err(err);
     } else if (ps.kind == PASS_MFMA_TILE) {
         MfmaTileArgs m = ps.mt;
@@ -841,9 +849,11 @@ void free_plan(PlanImpl *P)
 
 // Real-input plan: x (even, x/2 <= 4096) by y by z (y, z <= 4096), all powers
 // of two.  buf[0] holds the real array (x y z doubles, read as (x/2) y z
-// complex), buf[1] the half spectrum ((x/2 + 1) y z complex).  FORWARD (R2C):
-// m-point rows in buf[0], post-processing into buf[1], then Y and Z in
-// buf[1]; INVERSE (C2R) runs the same passes in reverse order, the
+// complex), buf[1] the half spectrum: rows of x/2 + 1 elements at a pitch of
+// x/2 + 8 elements (128-byte aligned rows for the strided passes).  FORWARD
+// (R2C): m-point rows in buf[0], post-processing into buf[1], then Y and Z
+// in buf[1] -- the columns 0..m-1 in 4-column tiles, the Nyquist column m
+// on its own; INVERSE (C2R) runs the same passes in reverse order, the
 // pre-processing scaled by 1/(m y z) so the result carries 1/(x y z).
 int build_real_plan(struct topeFFT *f, long long nx, long long ny, long long nz, const double *real, PlanImpl **out,
                     tope_ulong *pre_ns, tope_ulong *mem_ns)
@@ -865,12 +875,14 @@ int build_real_plan(struct topeFFT *f, long long nx, long long ny, long long nz,
     P->dims[0] = nx;
     P->dims[1] = ny;
     P->dims[2] = nz;
-    P->total = (m + 1) * rows;  // elements of the larger (spectrum) buffer
-    P->norm = 1;                // axis passes unscaled; the C2R pre-pass scales
+    const long long pitch = m + 8;
+    P->total = pitch * rows;  // elements of the larger (spectrum) buffer
+    P->norm = 1;              // axis passes unscaled; the C2R pre-pass scales
     P->bytes = (size_t)P->total * P->elem;
     P->real = true;
     P->real_m = m;
     P->real_rows = rows;
+    P->real_pitch = pitch;
     P->real_scale = 1.0 / ((double)m * (double)rows);
     P->real_bytes = (size_t)(nx * rows) * sizeof(double);
     hipStream_t s = fw->stream;
@@ -903,9 +915,22 @@ int build_real_plan(struct topeFFT *f, long long nx, long long ny, long long nz,
         ps.name = "real_half_spectrum";
         P->passes.push_back(ps);
         cur = 1;
-        const long long mp = m + 1;
-        rc = pl.axis(ny, Geo{nz, mp * ny, mp, 1, mp}, &cur, 1);
-        if (!rc) rc = pl.axis(nz, Geo{1, 0, mp * ny, 1, mp * ny}, &cur, 2);
+        // Y: columns kx < m of every z plane, then the Nyquist column kx = m
+        // (a single-column pass based at element m); Z likewise per y row
+        auto axis2 = [&](long long n, Geo g, int id) {
+            if (n == 1) return (int)TOPE_SUCCESS;
+            int r = pl.axis(n, g, &cur, id);
+            if (r) return r;
+            g.C = 1;
+            const size_t before = P->passes.size();
+            r = pl.axis(n, g, &cur, -1);
+            if (r) return r;
+            if (P->passes.size() != before + 1) return (int)TOPE_ERROR_UNSUPPORTED;
+            P->passes.back().base = m;
+            return (int)TOPE_SUCCESS;
+        };
+        rc = axis2(ny, Geo{nz, pitch * ny, m, 1, pitch}, 1);
+        if (!rc) rc = axis2(nz, Geo{ny, pitch, m, 1, pitch * ny}, 2);
         if (!rc && cur != 1) rc = TOPE_ERROR_UNSUPPORTED;
     }
     if (rc) return fail(rc);
@@ -1509,6 +1534,7 @@ void topeRealPlanInit(struct topeFFT *f, struct topePlanReal *t, int x, int y, i
     t->data = P->buf[1];
     t->realSize = P->real_bytes;
     t->dataSize = P->bytes;
+    t->pitch = (int)P->real_pitch;
 }
 
 void topeRealExec(struct topeFFT *f, struct topePlanReal *t, double *host, int dir)
@@ -1535,9 +1561,16 @@ void topeRealExec(struct topeFFT *f, struct topePlanReal *t, double *host, int d
     if (!rc) {
         EventPair dn;
         err = time_start(dn, s);
-        if (err == hipSuccess)
-            err = hipMemcpyAsync(host, dir == FORWARD ? P->buf[1] : P->buf[0],
-                                 dir == FORWARD ? P->bytes : P->real_bytes, hipMemcpyDeviceToHost, s);
+        if (err == hipSuccess) {
+            if (dir == FORWARD) {
+                // rows of m + 1 elements out of the padded device pitch
+                const size_t row = (size_t)(P->real_m + 1) * 16, dpitch = (size_t)P->real_pitch * 16;
+                err = hipMemcpy2DAsync(host, row, P->buf[1], dpitch, row, (size_t)P->real_rows, hipMemcpyDeviceToHost,
+                                       s);
+            } else {
+                err = hipMemcpyAsync(host, P->buf[0], P->real_bytes, hipMemcpyDeviceToHost, s);
+            }
+        }
         if (err == hipSuccess) err = time_stop(dn, s, &t->totalMemory);
         (void)hipEventDestroy(dn.start);
         (void)hipEventDestroy(dn.stop);

@@ -65,10 +65,11 @@ size_t mfma_constants_bytes();
 bool mfma_supported(int n);
 hipError_t launch_mfma_rows(int mode, int n, const void *in, void *out, const MfmaArgs &a, int sign, hipStream_t s);
 // Real-input transforms (kernels_real.hip): dir 0 = R2C half-spectrum
-// post-processing (Z, pitch m -> X, pitch m + 1), 1 = C2R pre-processing
-// (X -> Z, scaled by s); w = W_n^k, k = 0..m, n = 2m.
-hipError_t launch_real_pass(int dir, const void *in, void *out, const void *w, long long m, long long rows, double s,
-                            hipStream_t st);
+// post-processing (Z, row pitch m -> X, m + 1 values per row at row pitch
+// `pitch`), 1 = C2R pre-processing (X -> Z, scaled by s); w = W_n^k,
+// k = 0..m, n = 2m.
+hipError_t launch_real_pass(int dir, const void *in, void *out, const void *w, long long m, long long pitch,
+                            long long rows, double s, hipStream_t st);
 // Mixed-precision column tiles (mfma_tile.hpp): 16 unit-stride lines per block.
 struct MfmaTileArgs;
 bool mfma_tile_supported(int n);

@@ -17,16 +17,16 @@
 namespace tope {
 
 // One thread per mirror pair (k, m - k), k = 0..m/2, of one row: forward
-// reads Z (row pitch m) and writes X (row pitch m + 1).
+// reads Z (row pitch m) and writes X (m + 1 values per row, row pitch p).
 __global__ void r2c_post_kernel(const cpx<double> *__restrict__ Z, cpx<double> *__restrict__ X,
-                                const cpx<double> *__restrict__ w, long long m, long long rows)
+                                const cpx<double> *__restrict__ w, long long m, long long p, long long rows)
 {
     const long long pairs = m / 2 + 1;
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= pairs * rows) return;
     const long long row = idx / pairs, k = idx % pairs;
     const cpx<double> *z = Z + row * m;
-    cpx<double> *x = X + row * (m + 1);
+    cpx<double> *x = X + row * p;
     const cpx<double> zk = z[k % m], zm = z[(m - k) % m];
     // X[k] from a = Z[k], b = conj Z[m-k]; X[m-k] from a' = Z[m-k], b' = conj Z[k]
     auto out = [&](cpx<double> a, cpx<double> bc, long long kk) {
@@ -41,15 +41,15 @@ __global__ void r2c_post_kernel(const cpx<double> *__restrict__ Z, cpx<double> *
     if (m - k != k) out(zm, zk, m - k);
 }
 
-// Inverse: X (pitch m + 1) -> Z (pitch m), scaled by s; one thread per pair.
+// Inverse: X (row pitch p) -> Z (row pitch m), scaled by s; one thread per pair.
 __global__ void c2r_pre_kernel(const cpx<double> *__restrict__ X, cpx<double> *__restrict__ Z,
-                               const cpx<double> *__restrict__ w, long long m, long long rows, double s)
+                               const cpx<double> *__restrict__ w, long long m, long long p, long long rows, double s)
 {
     const long long pairs = m / 2 + 1;
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= pairs * rows) return;
     const long long row = idx / pairs, k = idx % pairs;
-    const cpx<double> *x = X + row * (m + 1);
+    const cpx<double> *x = X + row * p;
     cpx<double> *z = Z + row * m;
     const cpx<double> xk = x[k], xm = x[m - k];
     auto out = [&](cpx<double> a, cpx<double> bc, long long kk) {
@@ -64,18 +64,18 @@ __global__ void c2r_pre_kernel(const cpx<double> *__restrict__ X, cpx<double> *_
     if (m - k != k && m - k < m) out(xm, xk, m - k);
 }
 
-hipError_t launch_real_pass(int dir, const void *in, void *out, const void *w, long long m, long long rows, double s,
-                            hipStream_t st)
+hipError_t launch_real_pass(int dir, const void *in, void *out, const void *w, long long m, long long pitch,
+                            long long rows, double s, hipStream_t st)
 {
     const long long threads = (m / 2 + 1) * rows;
     const long long grid = (threads + 255) / 256;
     if (grid > 0x7fffffffLL) return hipErrorInvalidValue;
     if (dir == 0)
         hipLaunchKernelGGL(r2c_post_kernel, dim3((unsigned)grid), dim3(256), 0, st, static_cast<const cpx<double> *>(in),
-                           static_cast<cpx<double> *>(out), static_cast<const cpx<double> *>(w), m, rows);
+                           static_cast<cpx<double> *>(out), static_cast<const cpx<double> *>(w), m, pitch, rows);
     else
         hipLaunchKernelGGL(c2r_pre_kernel, dim3((unsigned)grid), dim3(256), 0, st, static_cast<const cpx<double> *>(in),
-                           static_cast<cpx<double> *>(out), static_cast<const cpx<double> *>(w), m, rows, s);
+                           static_cast<cpx<double> *>(out), static_cast<const cpx<double> *>(w), m, pitch, rows, s);
     return hipGetLastError();
 }
 

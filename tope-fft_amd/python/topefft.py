@@ -69,7 +69,7 @@ class TopePlanMany(C.Structure):
 
 class TopePlanReal(C.Structure):
     _fields_ = [("x", C.c_int), ("y", C.c_int), ("z", C.c_int), ("dim", C.c_int),
-                ("data", C.c_void_p), ("real", C.c_void_p), ("dataSize", C.c_size_t), ("realSize", C.c_size_t),
+                ("pitch", C.c_int), ("reserved", C.c_int), ("data", C.c_void_p), ("real", C.c_void_p), ("dataSize", C.c_size_t), ("realSize", C.c_size_t),
                 ("totalMemory", C.c_uint64), ("totalPreKernel", C.c_uint64), ("totalKernel", C.c_uint64),
                 ("impl", C.c_void_p)]
 
