@@ -246,6 +246,19 @@ void tope3DPlanInitMode(struct topeFFT *f, struct topePlan3D *t, int x, int y, i
 void tope2DExecEx(struct topeFFT *f, struct topePlan2D *t, void *host, int dir);
 void tope3DExecEx(struct topeFFT *f, struct topePlan3D *t, void *host, int dir);
 
+/* ---- caller streams and caller-owned device arrays (no reference
+ * counterpart) ------------------------------------------------------------
+ * topeFFTSetStream: every later operation of the framework's plans is
+ * enqueued on `stream` (a hipStream_t; NULL restores the framework's own
+ * stream).  Work already queued on the previous stream is ordered before
+ * it.  Batch plans (topeMany*) keep the stream they were created with.
+ * topePlanExecPtr: transforms the caller's device array `data` (the plan's
+ * element type and layout, interleaved complex) in place, asynchronously on
+ * the framework's stream, without host copies; plan_impl is the plan
+ * struct's impl field.  Not for real-input plans. */
+void topeFFTSetStream(struct topeFFT *f, void *stream);
+void topePlanExecPtr(struct topeFFT *f, void *plan_impl, void *data, int dir);
+
 /* ---- real-input transforms (no reference counterpart; the reference
  * declares R2C but runs C2C) ---------------------------------------------
  * x (even, x/2 <= 4096) by y by z (<= 4096 each; 1 for unused axes), powers

@@ -52,6 +52,44 @@ def test_many_fp64_all_methods(oracle):
         assert np.array_equal(b, back), topefft.MANY_METHOD_NAMES[m]
 
 
+def poison_device_memory(mib=256):
+    # a plan whose device arrays hold NaN, destroyed: its memory goes back to
+    # the allocator, where the next plan's allocations are likely to find it
+    n = mib * 2**20 // 16
+    fw = topefft.Framework()
+    p = topefft.Plan(fw, (n,), np.full(n, np.nan + 1j * np.nan))
+    p.exec_device(topefft.FORWARD)
+    fw.synchronize()
+    p.destroy()
+    fw.destroy()
+
+
+@pytest.mark.parametrize("prec", [topefft.PREC_F64, topefft.PREC_F32], ids=["fp64", "c64"])
+def test_many_independent_of_stale_device_memory(oracle, prec):
+    # every output element is written from this plan's data alone: after
+    # NaN-filled allocations were freed, all transforms (not only sampled
+    # ones) match numpy's FFT and the round trip restores the input
+    poison_device_memory()
+    sizes, x = make_batch(600, 0x57A1E)
+    if prec == topefft.PREC_F32:
+        x = x.astype(np.complex64)
+    tol = 1e-12 if prec == topefft.PREC_F64 else 1e-6
+    fw = topefft.Framework()
+    q = topefft.ManyPlan(fw, sizes, x, prec)
+    q.set_method(topefft.MANY_PERSISTENT)
+    fwd = q.exec(topefft.FORWARD).astype(np.complex128)
+    back = q.exec(topefft.INVERSE).astype(np.complex128)
+    q.destroy()
+    fw.destroy()
+    assert np.all(np.isfinite(fwd)) and np.all(np.isfinite(back))
+    x64 = x.astype(np.complex128)
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    worst = max(rel_l2(fwd[o:o + n], np.fft.fft(x64[o:o + n])) for n, o in zip(sizes, offs))
+    assert worst < tol
+    assert rel_l2(back, x64) < tol
+    check_against_oracle(oracle, sizes, x64, fwd, 1, per_class=1, tol=tol)
+
+
 def test_many_radix8_schedules_from_wisdom(oracle, tmp_path, monkeypatch):
     # force radix-8 stages for every class that allows them through a wisdom
     # file, as the autotuner would write it

@@ -39,7 +39,8 @@ struct EventPair {
 };
 
 struct FrameworkImpl {
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // the stream every plan op is enqueued on
+    hipStream_t own = nullptr;     // the framework's own stream (stream unless a caller's is set)
     int live_plans = 0;
     EventPair last;  // last exec's device time (profileThis)
 };
@@ -141,10 +142,13 @@ void ref_radix(long long n, int *radix, int *stages)
 // Timing events skip the system-scope fence (cache writeback/invalidate)
 // that a default event performs when it completes: with it, every event
 // between two passes costs ~10 us of idle GPU.
-hipError_t ensure_event(hipEvent_t *e)
+// Events that bracket work whose results a host copy reads next keep the
+// system-scope fence; timing-only events (per-pass timing) drop it, since a
+// fence between back-to-back kernels leaves an idle gap of several us.
+hipError_t ensure_event(hipEvent_t *e, bool fenced = true)
 {
     if (*e) return hipSuccess;
-    return hipEventCreateWithFlags(e, hipEventDisableSystemFence);
+    return hipEventCreateWithFlags(e, fenced ? hipEventDefault : hipEventDisableSystemFence);
 }
 
 const AxisVariant *find_variant_named(const char *name)
@@ -570,10 +574,10 @@ long long dif2_min_rows()
     return 1LL << 62;
 }
 
-hipError_t time_start(EventPair &ev, hipStream_t s)
+hipError_t time_start(EventPair &ev, hipStream_t s, bool fenced = true)
 {
-    hipError_t e = ensure_event(&ev.start);
-    if (e == hipSuccess) e = ensure_event(&ev.stop);
+    hipError_t e = ensure_event(&ev.start, fenced);
+    if (e == hipSuccess) e = ensure_event(&ev.stop, fenced);
     if (e == hipSuccess) e = hipEventRecord(ev.start, s);
     return e;
 }
@@ -598,7 +602,7 @@ hipEvent_t pool_event(PlanImpl *P)
 {
     if (P->pool_used == P->pool.size()) {
         hipEvent_t e = nullptr;
-        if (ensure_event(&e) != hipSuccess) return nullptr;
+        if (ensure_event(&e, false) != hipSuccess) return nullptr;
         P->pool.push_back(e);
     }
     return P->pool[P->pool_used++];
@@ -956,9 +960,10 @@ void release_framework(struct topeFFT *f)
 {
     FrameworkImpl *fw = fw_of(f);
     if (!fw) return;
-    if (fw->stream) {
-        (void)hipStreamSynchronize(fw->stream);
-        (void)hipStreamDestroy(fw->stream);
+    if (fw->stream) (void)hipStreamSynchronize(fw->stream);
+    if (fw->own) {
+        (void)hipStreamSynchronize(fw->own);
+        (void)hipStreamDestroy(fw->own);
     }
     if (fw->last.start) (void)hipEventDestroy(fw->last.start);
     if (fw->last.stop) (void)hipEventDestroy(fw->last.stop);
@@ -1048,12 +1053,13 @@ void topeFFTInit(struct topeFFT *f)
         f->error = TOPE_ERROR_UNSUPPORTED;
         return;
     }
-    err = hipStreamCreateWithFlags(&fw->stream, hipStreamNonBlocking);
+    err = hipStreamCreateWithFlags(&fw->own, hipStreamNonBlocking);
     if (err != hipSuccess) {
         delete fw;
         f->error = map_err(err);
         return;
     }
+    fw->stream = fw->own;
     f->impl = fw;
     f->stream = fw->stream;
     f->event = &fw->last;
@@ -1496,7 +1502,7 @@ double topePlanTimePass(struct topeFFT *f, void *plan_impl, int pass, int dir, i
         return -1.0;
     }
     EventPair ev;
-    hipError_t err = time_start(ev, fw->stream);
+    hipError_t err = time_start(ev, fw->stream, false);
     int rc = TOPE_SUCCESS;
     for (int r = 0; r < reps && err == hipSuccess && !rc; ++r) rc = launch_pass(P, pass, fw->stream, dir);
     tope_ulong ns = 0;
@@ -1593,6 +1599,58 @@ void topeRealDestroy(struct topeFFT *f, struct topePlanReal *t)
     t->impl = nullptr;
     t->data = t->real = nullptr;
     f->error = TOPE_SUCCESS;
+}
+
+/* ---- caller streams and caller-owned device arrays ----------------------- */
+
+void topeFFTSetStream(struct topeFFT *f, void *stream)
+{
+    if (!f) return;
+    FrameworkImpl *fw = fw_of(f);
+    if (!fw) {
+        f->error = TOPE_ERROR_NOT_INITIALIZED;
+        return;
+    }
+    hipStream_t next = stream ? static_cast<hipStream_t>(stream) : fw->own;
+    int rc = TOPE_SUCCESS;
+    if (next != fw->stream) {
+        // work already queued on the old stream completes before anything
+        // enqueued on the new one
+        hipEvent_t ev = nullptr;
+        hipError_t err = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (err == hipSuccess) err = hipEventRecord(ev, fw->stream);
+        if (err == hipSuccess) err = hipStreamWaitEvent(next, ev, 0);
+        if (ev) (void)hipEventDestroy(ev);
+        rc = map_err(err);
+        if (!rc) fw->stream = next;
+    }
+    f->stream = fw->stream;
+    f->error = rc;
+}
+
+void topePlanExecPtr(struct topeFFT *f, void *plan_impl, void *data, int dir)
+{
+    if (!f) return;
+    PlanImpl *P = impl_of(plan_impl);
+    FrameworkImpl *fw = fw_of(f);
+    if (!P || !fw || P->fw != fw) {
+        f->error = TOPE_ERROR_NOT_INITIALIZED;
+        return;
+    }
+    if (!data || P->real) {
+        f->error = !data ? TOPE_ERROR_INVALID_ARG : TOPE_ERROR_UNSUPPORTED;
+        return;
+    }
+    // run the plan's passes with the caller's array in place of its own;
+    // schedules that end in the scratch buffer copy the result back
+    void *own0 = P->buf[0], *own1 = P->buf[1];
+    P->buf[0] = data;
+    int rc = enqueue(P, fw->stream, dir);
+    if (!rc && P->final_buf == 1)
+        rc = map_err(hipMemcpyAsync(data, P->buf[0], P->bytes, hipMemcpyDeviceToDevice, fw->stream));
+    P->buf[0] = own0;
+    P->buf[1] = own1;
+    f->error = rc;
 }
 
 size_t topeStructSize(int which)

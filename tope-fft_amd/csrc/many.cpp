@@ -48,7 +48,8 @@ struct ManyImpl {
     int *d_ident = nullptr;
     // graphs per (method, dir)
     hipGraphExec_t gexec[4][2] = {};
-    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;  // timing only (autotuner): no system fence
+    hipEvent_t x0 = nullptr, x1 = nullptr;  // around exec-with-download: fenced
 };
 
 int map_err(hipError_t e) { return e == hipSuccess ? TOPE_SUCCESS : (int)e; }
@@ -228,6 +229,8 @@ void free_many(ManyImpl *M)
     if (M->d_ident) (void)hipFree(M->d_ident);
     if (M->e0) (void)hipEventDestroy(M->e0);
     if (M->e1) (void)hipEventDestroy(M->e1);
+    if (M->x0) (void)hipEventDestroy(M->x0);
+    if (M->x1) (void)hipEventDestroy(M->x1);
     delete M;
 }
 
@@ -479,20 +482,21 @@ void topeManyExec(struct topeFFT *f, struct topePlanMany *t, void *host, int dir
         f->error = TOPE_ERROR_INVALID_ARG;
         return;
     }
-    if (!M->e0 && hipEventCreateWithFlags(&M->e0, hipEventDisableSystemFence) != hipSuccess) {
+    // the download below reads what the kernels wrote: fenced events
+    if (!M->x0 && hipEventCreate(&M->x0) != hipSuccess) {
         f->error = TOPE_ERROR_UNSUPPORTED;
         return;
     }
-    if (!M->e1 && hipEventCreateWithFlags(&M->e1, hipEventDisableSystemFence) != hipSuccess) {
+    if (!M->x1 && hipEventCreate(&M->x1) != hipSuccess) {
         f->error = TOPE_ERROR_UNSUPPORTED;
         return;
     }
-    hipError_t err = hipEventRecord(M->e0, M->stream);
+    hipError_t err = hipEventRecord(M->x0, M->stream);
     int rc = err == hipSuccess ? launch_method(M, M->method, dir, M->stream) : map_err(err);
-    if (!rc) rc = map_err(hipEventRecord(M->e1, M->stream));
-    if (!rc) rc = map_err(hipEventSynchronize(M->e1));
+    if (!rc) rc = map_err(hipEventRecord(M->x1, M->stream));
+    if (!rc) rc = map_err(hipEventSynchronize(M->x1));
     float ms = 0;
-    if (!rc && hipEventElapsedTime(&ms, M->e0, M->e1) == hipSuccess) t->totalKernel += (tope_ulong)(ms * 1e6);
+    if (!rc && hipEventElapsedTime(&ms, M->x0, M->x1) == hipSuccess) t->totalKernel += (tope_ulong)(ms * 1e6);
     if (!rc)
         rc = map_err(hipMemcpyAsync(host, M->data, (size_t)M->elems * M->elem, hipMemcpyDeviceToHost, M->stream));
     if (!rc) rc = map_err(hipStreamSynchronize(M->stream));

@@ -89,6 +89,8 @@ _PB = C.POINTER(TopePlanBatch)
 _PM = C.POINTER(TopePlanMany)
 _PR = C.POINTER(TopePlanReal)
 SIGNATURES = {
+    "topeFFTSetStream": (None, [_F, C.c_void_p]),
+    "topePlanExecPtr": (None, [_F, C.c_void_p, C.c_void_p, C.c_int]),
     "topeRealPlanInit": (None, [_F, _PR, C.c_int, C.c_int, C.c_int, _DP]),
     "topeRealExec": (None, [_F, _PR, _DP, C.c_int]),
     "topeRealExecDevice": (None, [_F, _PR, C.c_int]),
@@ -193,6 +195,13 @@ class Framework:
         self.lib.topeFFTSynchronize(C.byref(self.f))
         self.check("topeFFTSynchronize")
 
+    def set_stream(self, stream):
+        """Enqueue later plan operations on a caller's hipStream_t (an int
+        handle, e.g. torch.cuda.current_stream().cuda_stream); None restores
+        the framework's own stream."""
+        self.lib.topeFFTSetStream(C.byref(self.f), C.c_void_p(stream) if stream else None)
+        self.check("topeFFTSetStream")
+
     def destroy(self):
         if self.f.impl:
             self.lib.topeFFTDestroy(C.byref(self.f))
@@ -276,6 +285,12 @@ class Plan:
         fn = getattr(self.lib, f"tope{self.ndim}DExecDevice")
         fn(C.byref(self.fw.f), C.byref(self.t), direction)
         self.fw.check(f"tope{self.ndim}DExecDevice")
+
+    def exec_ptr(self, device_ptr, direction=FORWARD):
+        """Transform a caller-owned device array (the plan's layout) in place,
+        asynchronously on the framework's stream."""
+        self.lib.topePlanExecPtr(C.byref(self.fw.f), self.impl, C.c_void_p(device_ptr), direction)
+        self.fw.check("topePlanExecPtr")
 
     def upload(self, x):
         h = np.ascontiguousarray(x, dtype=self.dtype).reshape(-1)
