@@ -10,11 +10,16 @@ import topefft
 from conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
-torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def torch():
+    # imported only when a GPU test runs (not at collection)
+    return pytest.importorskip("torch")
 
 
 @pytest.mark.parametrize("dims", [(64, 32), (1 << 15,), (16, 8, 4)], ids=lambda d: "x".join(map(str, d)))
-def test_exec_ptr_torch_stream(oracle, dims):
+def test_exec_ptr_torch_stream(oracle, torch, dims):
     n = int(np.prod(dims))
     rng = np.random.default_rng(n)
     x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
@@ -37,7 +42,7 @@ def test_exec_ptr_torch_stream(oracle, dims):
     assert rel_l2(back, x) < 1e-12
 
 
-def test_exec_ptr_complex64(oracle):
+def test_exec_ptr_complex64(oracle, torch):
     dims = (256, 256)
     n = 256 * 256
     rng = np.random.default_rng(5)
