@@ -79,6 +79,25 @@ def test_1d_non_pow2_dft(gfw, oracle, n):
         assert rel_l2(got, oracle.dft(x, (n,), d)) < TOL, (n, d)
 
 
+@pytest.mark.parametrize("n", [65, 4097, 10007, 12000, 30000])
+def test_1d_bluestein(oracle, n):
+    # non-power-of-two lengths above 64 run Bluestein's chirp-z convolution
+    # over power-of-two passes (length M >= 2n - 1, four-step above 4096)
+    x = seeded((n,))
+    fw = topefft.Framework()
+    p = topefft.Plan(fw, (n,), x)
+    names = p.pass_names()
+    fwd = p.exec(topefft.FORWARD)
+    back = p.exec(topefft.INVERSE)
+    p.destroy()
+    fw.destroy()
+    assert names[0] == "chirp_pre" and names[-1] == "chirp_post", names
+    assert rel_l2(fwd, oracle.dft(x, (n,), topefft.FORWARD)) < TOL, n
+    assert rel_l2(back, x) < TOL, n
+    if n <= 12000:
+        assert rel_l2(topefft.transform(x, (n,), topefft.INVERSE), oracle.dft(x, (n,), topefft.INVERSE)) < TOL
+
+
 def test_1d_golden_fixtures(gfw, golden):
     import test_oracle as T
     for n in [2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 3, 5, 7, 12, 17, 100]:

@@ -32,6 +32,7 @@ namespace {
 
 constexpr int kMaxFused = 4096;          // longest axis one pass holds on chip
 constexpr long long kMaxAxis = 1LL << 24;  // four-step limit (4096 x 4096)
+constexpr long long kChirpMin = 64;        // longer non-power-of-two 1D lengths use Bluestein
 
 struct EventPair {
     hipEvent_t start = nullptr, stop = nullptr;
@@ -45,7 +46,7 @@ struct FrameworkImpl {
     EventPair last;  // last exec's device time (profileThis)
 };
 
-enum PassKind { PASS_AXIS = 0, PASS_DFT = 1, PASS_MFMA = 2, PASS_MFMA_TILE = 3, PASS_REAL = 4 };
+enum PassKind { PASS_AXIS = 0, PASS_DFT = 1, PASS_MFMA = 2, PASS_MFMA_TILE = 3, PASS_REAL = 4, PASS_CHIRP = 5 };
 
 struct Pass {
     int kind = PASS_AXIS;
@@ -63,6 +64,10 @@ struct Pass {
     long long dft_n = 0;
     const void *dft_roots = nullptr;
     const void *real_w = nullptr;  // PASS_REAL: W_n^k, k = 0..m (FP64)
+    int fixed_dir = -1;            // >= 0: the pass always runs in this direction (Bluestein's inner FFTs)
+    int chirp_op = 0;              // PASS_CHIRP: CHIRP_PRE / CHIRP_MUL / CHIRP_POST
+    long long chirp_n = 0, chirp_M = 0;
+    const void *chirp_tab[2] = {nullptr, nullptr};  // transformed filters, forward / inverse
     const char *name = "";
 };
 
@@ -74,7 +79,8 @@ struct PlanImpl {
     long long dims[3] = {1, 1, 1};
     long long total = 0;
     long long norm = 0;  // elements of one transform: INVERSE divides by it
-    size_t bytes = 0;
+    size_t bytes = 0;      // the transform's data (host transfers)
+    size_t buf_bytes = 0;  // device arrays when larger than bytes (Bluestein's padded length)
     void *buf[2] = {nullptr, nullptr};
     int final_buf = 0;
     std::vector<Pass> passes;
@@ -239,6 +245,8 @@ const AxisVariant *select_variant(int prec, int n, bool cfast, long long lines, 
     }
     return best;
 }
+
+int launch_pass(PlanImpl *P, size_t i, hipStream_t s, int dir);
 
 struct Geo {  // lines (o < O, c < C) of length n, element stride is
     long long O, os, C, cs, is;
@@ -458,6 +466,60 @@ class Planner {
         return TOPE_SUCCESS;
     }
 
+    // Non-power-of-two length n by Bluestein's identity (kernels_chirp.hip):
+    // chirp pre-multiply into buffer 1 (length M, zero padded), forward
+    // FFT_M, product with the transformed filter of the direction, inverse
+    // FFT_M (both inner FFTs keep their direction whatever the outer one),
+    // chirp post-multiply into buffer 0.  The filters are transformed here,
+    // at plan time, by the same FFT_M passes.
+    int bluestein(long long n, long long M)
+    {
+        const size_t tb = (size_t)M * 16;
+        void *tab[2] = {nullptr, nullptr};
+        for (int d = 0; d < 2; ++d) {
+            hipError_t err = hipMalloc(&tab[d], tb);
+            if (err != hipSuccess) return map_err(err);
+            P->allocs.push_back(tab[d]);
+        }
+        auto chirp_pass = [&](int op, int bi, int bo) {
+            Pass ps;
+            ps.kind = PASS_CHIRP;
+            ps.chirp_op = op;
+            ps.chirp_n = n;
+            ps.chirp_M = M;
+            ps.chirp_tab[0] = tab[0];
+            ps.chirp_tab[1] = tab[1];
+            ps.in_buf = bi;
+            ps.out_buf = bo;
+            ps.name = op == CHIRP_PRE ? "chirp_pre" : op == CHIRP_MUL ? "chirp_mul" : "chirp_post";
+            P->passes.push_back(ps);
+        };
+        chirp_pass(CHIRP_PRE, 0, 1);
+        int cur = 1;
+        const size_t f0 = P->passes.size();
+        int rc = axis(M, Geo{1, 0, 1, M, 1}, &cur, 0);
+        if (rc) return rc;
+        const size_t f1 = P->passes.size();
+        for (size_t i = f0; i < f1; ++i) P->passes[i].fixed_dir = FORWARD;
+        const int after_fwd = cur;
+        chirp_pass(CHIRP_MUL, cur, cur);
+        rc = axis(M, Geo{1, 0, 1, M, 1}, &cur, -1);
+        if (rc) return rc;
+        for (size_t i = f1 + 1; i < P->passes.size(); ++i) P->passes[i].fixed_dir = INVERSE;
+        chirp_pass(CHIRP_POST, cur, 0);
+
+        // transformed filters: b into buffer 1, the forward FFT_M passes,
+        // then the result (in buffer after_fwd) into the table
+        for (int d = 0; d < 2; ++d) {
+            hipError_t err = launch_chirp(CHIRP_FILTER, nullptr, P->buf[1], nullptr, n, M, 1.0, d, st);
+            for (size_t i = f0; i < f1 && err == hipSuccess; ++i)
+                if (launch_pass(P, i, st, FORWARD)) err = hipErrorUnknown;
+            if (err == hipSuccess) err = hipMemcpyAsync(tab[d], P->buf[after_fwd], tb, hipMemcpyDeviceToDevice, st);
+            if (err != hipSuccess) return map_err(err);
+        }
+        return TOPE_SUCCESS;
+    }
+
     // DFT-16 operand constants of the MFMA passes (one table per plan).
     hipError_t mfma_consts(const MfmaConsts **out)
     {
@@ -611,9 +673,9 @@ hipEvent_t pool_event(PlanImpl *P)
 // Launch pass i of a plan on stream s.
 int launch_pass(PlanImpl *P, size_t i, hipStream_t s, int dir)
 {
-    const int sign = (dir == FORWARD) ? -1 : 1;
-    const double sc = (dir == INVERSE) ? 1.0 / (double)P->norm : 1.0;
     const Pass &ps = P->passes[i];
+    const int sign = ((ps.fixed_dir >= 0 ? ps.fixed_dir : dir) == FORWARD) ? -1 : 1;
+    const double sc = (dir == INVERSE) ? 1.0 / (double)P->norm : 1.0;
     const char *ib = static_cast<const char *>(P->buf[ps.in_buf]);
     char *ob = static_cast<char *>(P->buf[ps.out_buf]);
     const long long eb = (long long)P->elem;
@@ -639,6 +701,12 @@ int launch_pass(PlanImpl *P, size_t i, hipStream_t s, int dir)
         MfmaArgs m = ps.m;
         m.scale = ps.last ? (float)sc : 1.0f;  // 1/N is a power of two: exact in FP32
         hipError_t err = launch_mfma_rows(ps.mfma_mode, ps.mfma_n, ib, ob, m, sign, s);
+        if (err != hipSuccess) return map_err(err);
+    } else if (ps.kind == PASS_CHIRP) {
+        const int inv = dir == INVERSE ? 1 : 0;
+        const double M = (double)ps.chirp_M, n = (double)ps.chirp_n;
+        const double post = inv ? 1.0 / (M * n) : 1.0 / M;
+        hipError_t err = launch_chirp(ps.chirp_op, ib, ob, ps.chirp_tab[inv], ps.chirp_n, ps.chirp_M, post, inv, s);
         if (err != hipSuccess) return map_err(err);
     } else if (ps.kind == PASS_REAL) {
         const bool fwd = dir == FORWARD;
@@ -731,6 +799,20 @@ int build_plan(struct topeFFT *f, int shape, long long nx, long long ny, long lo
     P->total = total;
     P->norm = shape == SHAPE_BATCH ? nx : total;
     P->bytes = (size_t)total * P->elem;
+    // non-power-of-two 1D above kChirpMin: Bluestein over the power of two
+    // M >= 2n - 1 (device arrays of M elements, transfers of n)
+    const bool chirpz = shape == 1 && !is_pow2(nx) && nx > kChirpMin;
+    long long chirp_M = 1;
+    if (chirpz) {
+        while (chirp_M < 2 * nx - 1) chirp_M <<= 1;
+        if (chirp_M > kMaxAxis) {
+            delete P;
+            return TOPE_ERROR_INVALID_SIZE;
+        }
+        P->total = chirp_M;  // element count the inner passes address
+        P->norm = 1;         // scales applied by the chirp post-pass
+    }
+    P->buf_bytes = chirpz ? (size_t)chirp_M * P->elem : P->bytes;
     hipStream_t s = fw->stream;
 
     auto fail = [&](int rc) {
@@ -741,8 +823,12 @@ int build_plan(struct topeFFT *f, int shape, long long nx, long long ny, long lo
         return rc;
     };
 
-    hipError_t err = hipMalloc(&P->buf[0], P->bytes);
+    hipError_t err = hipMalloc(&P->buf[0], P->buf_bytes);
     if (err != hipSuccess) return fail(map_err(err));
+    if (chirpz) {
+        err = hipMalloc(&P->buf[1], P->buf_bytes);
+        if (err != hipSuccess) return fail(map_err(err));
+    }
 
     EventPair pre;
     err = time_start(pre, s);
@@ -778,8 +864,10 @@ int build_plan(struct topeFFT *f, int shape, long long nx, long long ny, long lo
         P->axis_tw[0].table = tw;
     } else if (shape == SHAPE_BATCH) {
         rc = pl.axis(nx, Geo{1, 0, ny, nx, 1}, &cur, 0);
+    } else if (chirpz) {
+        rc = pl.bluestein(nx, chirp_M);
     } else if (shape == 1 && !is_pow2(nx)) {
-        // non-power-of-two 1D: GPU DFT with exactly reduced phases
+        // short non-power-of-two 1D: GPU DFT with exactly reduced phases
         void *roots = nullptr;
         err = hipMalloc(&roots, (size_t)nx * 16);
         if (err != hipSuccess) return fail(map_err(err));
@@ -817,8 +905,8 @@ int build_plan(struct topeFFT *f, int shape, long long nx, long long ny, long lo
     bool need_scratch = false;
     for (const Pass &ps : P->passes)
         if (ps.in_buf == 1 || ps.out_buf == 1) need_scratch = true;
-    if (need_scratch) {
-        err = hipMalloc(&P->buf[1], P->bytes);
+    if (need_scratch && !P->buf[1]) {
+        err = hipMalloc(&P->buf[1], P->buf_bytes);
         if (err != hipSuccess) return fail(map_err(err));
     }
     err = time_stop(pre, s, pre_ns);
@@ -1637,7 +1725,8 @@ void topePlanExecPtr(struct topeFFT *f, void *plan_impl, void *data, int dir)
         f->error = TOPE_ERROR_NOT_INITIALIZED;
         return;
     }
-    if (!data || P->real) {
+    if (!data || P->real || P->buf_bytes > P->bytes) {
+        // real plans and padded (Bluestein) plans work on arrays of their own
         f->error = !data ? TOPE_ERROR_INVALID_ARG : TOPE_ERROR_UNSUPPORTED;
         return;
     }
