@@ -60,10 +60,11 @@ def test_header_compiles_as_c(tmp_path):
                     str(tmp_path / "t")], check=True)
 
 
-def test_dropin_harness_builds_and_links(tmp_path):
-    exe = tmp_path / "harness2d"
-    subprocess.run(["gcc", "-std=gnu99", "-O2", "-Wall", "-I", os.path.dirname(HEADER),
-                    os.path.join(REPO, "tope-fft_amd", "examples", "harness2d.c"), "-o", str(exe),
+@pytest.mark.parametrize("name", ["harness1d", "harness2d", "harness3d", "harness_real"])
+def test_dropin_harness_builds_and_links(tmp_path, name):
+    exe = tmp_path / name
+    subprocess.run(["gcc", "-std=gnu99", "-O2", "-Wall", "-Werror", "-I", os.path.dirname(HEADER),
+                    os.path.join(REPO, "tope-fft_amd", "examples", f"{name}.c"), "-o", str(exe),
                     "-L", LIBDIR, "-ltopefft", "-lm", f"-Wl,-rpath,{LIBDIR}"], check=True)
     assert exe.exists()
 
