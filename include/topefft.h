@@ -195,7 +195,7 @@ const char *topePlanPassName(void *plan_impl, int pass);
  *   TOPE_MODE_MFMA_FP32   plain FP32 operands on MFMA
  *   TOPE_MODE_MFMA_FP16X3 2 fp16 parts, 3 products (~FP32 accuracy), each
  *                         transform scaled by a power of two into fp16 range
- * MFMA modes take n = 256 or 4096.  Twiddles and DFT-matrix constants are
+ * MFMA modes take n = 256, 512, 1024, 2048 or 4096.  Twiddles and DFT-matrix constants are
  * generated in FP64 on the GPU and rounded once; INVERSE scales by 1/n. */
 #define TOPE_PREC_F64 0
 #define TOPE_PREC_F32 1
@@ -235,7 +235,7 @@ void tope3DPlanInitEx(struct topeFFT *f, struct topePlan3D *t, int x, int y, int
                       const void *host);
 /* Complex64 2D/3D plans whose axis passes run on the matrix cores in the
  * given TOPE_MODE_MFMA_* operand mode where a kernel covers the axis:
- * contiguous rows of 256 or 4096, unit-stride column axes of 256, 512 or
+ * contiguous rows of 256 to 4096, unit-stride column axes of 256, 512 or
  * 1024 with a multiple of 16 lines; other axes take the FP32 VALU pass
  * (topePlanPassName tells which).  precision must be TOPE_PREC_F32 unless
  * mode is TOPE_MODE_VALU (then identical to the Ex functions). */

@@ -45,7 +45,7 @@ def test_batch_fp64(oracle, n):
 @pytest.mark.parametrize("mode", [topefft.MODE_VALU, topefft.MODE_MFMA_BF16X3, topefft.MODE_MFMA_BF16X6,
                                   topefft.MODE_MFMA_FP32, topefft.MODE_MFMA_FP16X3],
                          ids=lambda m: topefft.MODE_NAMES[m])
-@pytest.mark.parametrize("n", [256, 4096])
+@pytest.mark.parametrize("n", [256, 512, 1024, 2048, 4096])
 def test_batch_fp32_modes(oracle, n, mode):
     count = 40
     x = batch_input(n, count, 200 + n).astype(np.complex64)
@@ -78,7 +78,7 @@ def test_batch_fp32_full_4096(oracle):
         assert err < tol
 
 
-@pytest.mark.parametrize("n", [256, 4096])
+@pytest.mark.parametrize("n", [256, 512, 2048, 4096])
 def test_batch_fp16x3_range(oracle, n):
     # fp16 operands: every transform is scaled by its own power of two, so
     # magnitudes far outside fp16's range, mixed within one batch, keep the
@@ -122,9 +122,9 @@ def test_batch_round_trip_mfma():
 
 def test_batch_rejects_unsupported():
     fw = topefft.Framework()
-    x = np.zeros(512 * 2, np.complex64)
+    x = np.zeros(128 * 2, np.complex64)
     with pytest.raises(topefft.TopeError) as e:
-        topefft.BatchPlan(fw, 512, 2, x, topefft.PREC_F32, topefft.MODE_MFMA_BF16X6)
+        topefft.BatchPlan(fw, 128, 2, x, topefft.PREC_F32, topefft.MODE_MFMA_BF16X6)
     assert e.value.code == topefft.TOPE_ERROR_UNSUPPORTED
     with pytest.raises(topefft.TopeError):
         topefft.BatchPlan(fw, 256, 2, np.zeros(512, complex), topefft.PREC_F64, topefft.MODE_MFMA_FP32)

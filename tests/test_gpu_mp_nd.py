@@ -29,6 +29,10 @@ SHAPES = [
     ((16, 16, 512), ["axis_float", "axis_float", "mfma_cols"]),
     ((32, 256, 16), ["axis_float", "mfma_cols", "axis_float"]),
     ((16, 16, 1024), ["axis_float", "axis_float", "mfma_cols"]),
+    ((512, 32), ["mfma_rows", "axis_float"]),
+    ((1024, 16), ["mfma_rows", "axis_float"]),
+    ((2048, 16), ["mfma_rows", "axis_float"]),
+    ((512, 16, 512), ["mfma_rows", "axis_float", "mfma_cols"]),
 ]
 
 _want = {}

@@ -537,8 +537,8 @@ class Planner {
     }
 
     // Axis pass on the matrix cores when a kernel covers the geometry
-    // (complex64 storage): contiguous rows of n = 256 or 4096 back to back
-    // (row kernel), or unit-stride column tiles of 16 lines with n = 256,
+    // (complex64 storage): contiguous rows of n = 256 .. 4096 back to back
+    // (row kernels), or unit-stride column tiles of 16 lines with n = 256,
     // 512, 1024 (tile kernel).  Otherwise the FP32 VALU axis pass.
     int axis_mp(long long n, const Geo &g, int *cur, int axis_id, int mode)
     {

@@ -117,17 +117,37 @@ static hipError_t launch_n(const void *in, void *out, const MfmaArgs &a, int sig
     return launch_nw<MODE, N, 8>(in, out, a, sign, s);
 }
 
+template <int MODE, int N>
+static hipError_t launch_rem(const void *in, void *out, const MfmaArgs &a, int sign, hipStream_t s)
+{
+    using SH = MfmaRemShape<N>;
+    const long long blocks = (a.count + SH::TPB - 1) / SH::TPB;
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    const cpx<float> *i = static_cast<const cpx<float> *>(in);
+    cpx<float> *o = static_cast<cpx<float> *>(out);
+    if (sign < 0)
+        hipLaunchKernelGGL((mfma_rows_rem_kernel<MODE, N, -1>), dim3((unsigned)blocks), dim3(SH::THREADS), 0, s, i, o,
+                           a);
+    else
+        hipLaunchKernelGGL((mfma_rows_rem_kernel<MODE, N, 1>), dim3((unsigned)blocks), dim3(SH::THREADS), 0, s, i, o,
+                           a);
+    return hipGetLastError();
+}
+
 template <int MODE>
 static hipError_t launch_mode(int n, const void *in, void *out, const MfmaArgs &a, int sign, hipStream_t s)
 {
     switch (n) {
     case 256: return launch_n<MODE, 256>(in, out, a, sign, s);
+    case 512: return launch_rem<MODE, 512>(in, out, a, sign, s);
+    case 1024: return launch_rem<MODE, 1024>(in, out, a, sign, s);
+    case 2048: return launch_rem<MODE, 2048>(in, out, a, sign, s);
     case 4096: return launch_n<MODE, 4096>(in, out, a, sign, s);
     default: return hipErrorInvalidValue;
     }
 }
 
-bool mfma_supported(int n) { return n == 256 || n == 4096; }
+bool mfma_supported(int n) { return n == 256 || n == 512 || n == 1024 || n == 2048 || n == 4096; }
 
 template <int MODE, int N>
 static hipError_t launch_tile_n(const void *in, void *out, const MfmaTileArgs &a, unsigned grid, int sign,

@@ -384,4 +384,202 @@ __global__ void __launch_bounds__((MfmaShape<N, WMAX>::THREADS)) __attribute__((
     }
 }
 
+
+// ---- rows of N = R 16^2 (R = 2, 4, 8: N = 512, 1024, 2048) -------------
+// A leading radix-R stage on the VALU (thread t of a transform takes
+// butterflies t + T u), then the two radix-16 stages on MFMA as above.
+
+// LDS word of element i of a transform's slab: XOR of fixed masks selected
+// by index bits 5 and up into bits 0..5 (a bijection, linear over GF(2)),
+// found by an exhaustive bank model of this kernel's three access patterns
+// (radix-R outputs j R + r, stage inputs j + NB (4g + s), first-stage
+// outputs (j / R) 16 R + j % R + (4g + r) R): every access puts a wave's 64
+// lanes on 64 distinct banks and each half wave on 32.
+template <int N>
+__device__ __forceinline__ constexpr int rem_swz(int i)
+{
+    constexpr int m512[] = {21, 15, 48, 28, 0, 0};
+    constexpr int m1024[] = {11, 21, 31, 61, 26, 0};
+    constexpr int m2048[] = {19, 36, 9, 29, 21, 42};
+    const int *m = N == 512 ? m512 : N == 1024 ? m1024 : m2048;
+    int w = i;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) w ^= ((i >> (5 + b)) & 1) * m[b];
+    return w;
+}
+
+template <int N>
+struct MfmaRemShape {
+    using SC = Sched<N, 16>;
+    static constexpr int R = SC::REM;
+    static constexpr int NB = N / 16;                  // butterflies per MFMA stage
+    static constexpr int NG = NB / 16;                 // groups per MFMA stage
+    static constexpr int WPT = NG >= 4 ? 4 : NG;       // waves per transform
+    static constexpr int GPW = NG / WPT;               // groups per wave
+    static constexpr int TPB = WPT >= 4 ? 1 : 4 / WPT;  // transforms per block
+    static constexpr int T = 64 * WPT;                 // threads per transform
+    static constexpr int U = N / R / T;                // radix-R butterflies per thread
+    static constexpr int THREADS = 64 * WPT * TPB;
+    static_assert(R > 1 && SC::NST == 3 && U * R == 4 * GPW, "N = R 16^2, R = 2, 4, 8");
+};
+
+template <int MODE, int N, int SIGN>
+__global__ void __launch_bounds__((MfmaRemShape<N>::THREADS)) mfma_rows_rem_kernel(const cpx<float> *in,
+                                                                                   cpx<float> *out, const MfmaArgs a)
+{
+    using SH = MfmaRemShape<N>;
+    using SC = typename SH::SC;
+    constexpr int R = SH::R, NB = SH::NB, WPT = SH::WPT, GPW = SH::GPW, TPB = SH::TPB, T = SH::T, U = SH::U;
+    constexpr int CB = 8;
+    __shared__ float lds[TPB * N];
+    __shared__ float wave_max[SH::THREADS / 64];
+
+    const int tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    const int c = lane & 15, g = lane >> 4;
+    const int tb = wave / WPT, wt = wave % WPT;
+    const int t = wt * 64 + lane;  // thread within the transform
+    const long long row = (long long)blockIdx.x * TPB + tb;
+    const bool valid = row < a.count;
+    const int dir = SIGN < 0 ? 0 : 1;
+    float *slab = lds + tb * N;
+
+    // radix-R inputs: butterfly j = t + T u takes elements j + (N/R) m; the
+    // R U = 4 GPW values of a thread fill x[e / 4][e % 4] (re), [4 + e % 4] (im)
+    float x[GPW][8];
+    {
+        const __amdgpu_buffer_rsrc_t rin = make_rsrc(in + row * N, (a.in_elems - row * N) * CB);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int m = 0; m < R; ++m) {
+                const int e = u * R + m;
+                const cpx<float> v = valid ? buf_load<float>(rin, (t + T * u + (N / R) * m) * CB, 0) : cpx<float>{0, 0};
+                x[e / 4][e % 4] = v.x;
+                x[e / 4][4 + e % 4] = v.y;
+            }
+    }
+
+    float out_scale = a.scale;
+    if constexpr (MODE == MP_FP16X3) {
+        float m = 0.0f;
+#pragma unroll
+        for (int q = 0; q < GPW; ++q)
+#pragma unroll
+            for (int s = 0; s < 8; ++s) m = fmaxf(m, fabsf(x[q][s]));
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+        if constexpr (WPT > 1) {
+            if (lane == 0) wave_max[wave] = m;
+            __syncthreads();
+#pragma unroll
+            for (int w = 0; w < WPT; ++w) m = fmaxf(m, wave_max[tb * WPT + w]);
+        }
+        // the last stage's input is at most N/16 times the largest input
+        constexpr int TOP = 15 - ilog2c(N / 16);
+        int e = __builtin_amdgcn_frexp_expf(m);
+        e = e < TOP - 126 ? TOP - 126 : (e > TOP + 126 ? TOP + 126 : e);
+        const float up = __builtin_ldexpf(1.0f, TOP - e);
+        out_scale *= __builtin_ldexpf(1.0f, e - TOP);
+#pragma unroll
+        for (int q = 0; q < GPW; ++q)
+#pragma unroll
+            for (int s = 0; s < 8; ++s) x[q][s] *= up;
+    }
+
+    // radix-R stage (span 1: no twiddles); output r of butterfly j goes to j R + r
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        cpx<float> v[R];
+#pragma unroll
+        for (int m = 0; m < R; ++m) {
+            const int e = u * R + m;
+            v[m] = cpx<float>{x[e / 4][e % 4], x[e / 4][4 + e % 4]};
+        }
+        dft<R, SIGN>(v);
+#pragma unroll
+        for (int m = 0; m < R; ++m) {
+            const int e = u * R + m;
+            x[e / 4][e % 4] = v[m].x;
+            x[e / 4][4 + e % 4] = v[m].y;
+        }
+    }
+    // lane parts of the LDS words (every index is lane part ^ constant part
+    // in disjoint bits, and rem_swz is linear)
+    const int j0 = wt * GPW * 16 + c;  // butterfly of group q: j0 + 16 q
+    const int w_rem = rem_swz<N>(t * R);
+    const int w_in = rem_swz<N>(j0 + NB * 4 * g);
+    const int w_s1 = rem_swz<N>(((j0 / R) << ilog2c(16 * R)) + (j0 & (R - 1)) + 4 * g * R);
+#pragma unroll
+    for (int part = 0; part < 2; ++part) {
+        if (part > 0) __syncthreads();
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int e = u * R + r;
+                slab[w_rem ^ rem_swz<N>(T * R * u + r)] = x[e / 4][4 * part + e % 4];
+            }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < GPW; ++q)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) x[q][4 * part + s] = slab[w_in ^ rem_swz<N>(16 * q + NB * s)];
+    }
+
+#pragma unroll
+    for (int S = 1; S < 3; ++S) {
+        const int Ns = SC::span(S);
+#pragma unroll
+        for (int q = 0; q < GPW; ++q) {
+            const int j = j0 + 16 * q;
+            // input m = 4g + s of butterfly j: W_{16 Ns}^{(j mod Ns) m}
+            const cpx<float> *tw = a.tw + SC::twoff(S) + (j & (Ns - 1));
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int m = 4 * g + s;
+                cpx<float> w = tw[(m > 0 ? m - 1 : 0) * Ns];
+                if (m == 0) w = cpx<float>{1.0f, 0.0f};
+                const cpx<float> v = twmul<SIGN>(cpx<float>{x[q][s], x[q][4 + s]}, w);
+                x[q][s] = v.x;
+                x[q][4 + s] = v.y;
+            }
+            float y[8];
+            dft16_mfma<MODE>(x[q], y, a.k, dir, lane);
+#pragma unroll
+            for (int s = 0; s < 8; ++s) x[q][s] = y[s];
+        }
+        if (S == 2) {
+            if (valid) {
+                const __amdgpu_buffer_rsrc_t rout = make_rsrc(out + row * N, (a.out_elems - row * N) * CB);
+                constexpr bool SCALED = MODE == MP_FP16X3 || SIGN > 0;
+#pragma unroll
+                for (int q = 0; q < GPW; ++q)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        cpx<float> v{x[q][r], x[q][4 + r]};
+                        if constexpr (SCALED) v = scale(v, out_scale);
+                        buf_store<float>(v, rout, (j0 + 16 * q + NB * (4 * g + r)) * CB, 0);
+                    }
+            }
+        } else {
+            // output r of butterfly j (span R): (j / R) 16 R + j % R + (4g + r) R
+#pragma unroll
+            for (int part = 0; part < 2; ++part) {
+                __syncthreads();
+#pragma unroll
+                for (int q = 0; q < GPW; ++q)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        slab[w_s1 ^ rem_swz<N>(((16 * q / R) << ilog2c(16 * R)) + r * R)] = x[q][4 * part + r];
+                __syncthreads();
+#pragma unroll
+                for (int q = 0; q < GPW; ++q)
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) x[q][4 * part + s] = slab[w_in ^ rem_swz<N>(16 * q + NB * s)];
+            }
+        }
+    }
+}
+
 }  // namespace tope
