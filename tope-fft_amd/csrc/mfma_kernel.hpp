@@ -414,9 +414,11 @@ struct MfmaRemShape {
     static constexpr int R = SC::REM;
     static constexpr int NB = N / 16;                  // butterflies per MFMA stage
     static constexpr int NG = NB / 16;                 // groups per MFMA stage
-    static constexpr int WPT = NG >= 4 ? 4 : NG;       // waves per transform
+    // up to 4 groups per lane (16 complex values): few waves per transform
+    // and many transforms in flight per CU
+    static constexpr int WPT = NG >= 8 ? NG / 4 : 1;   // waves per transform
     static constexpr int GPW = NG / WPT;               // groups per wave
-    static constexpr int TPB = WPT >= 4 ? 1 : 4 / WPT;  // transforms per block
+    static constexpr int TPB = 4 / WPT;                // transforms per block (4 waves)
     static constexpr int T = 64 * WPT;                 // threads per transform
     static constexpr int U = N / R / T;                // radix-R butterflies per thread
     static constexpr int THREADS = 64 * WPT * TPB;
