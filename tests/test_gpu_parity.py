@@ -106,6 +106,20 @@ def test_1d_golden_fixtures(gfw, golden):
         assert rel_l2(run(gfw, (n,), x, 0), golden[f"inv_{n}"]) < TOL
 
 
+ND_NON_POW2 = [(12, 1), (3, 5), (100, 64), (64, 100), (30, 50), (2048 - 1, 4), (4, 1000), (12, 10, 8),
+               (16, 24, 5), (7, 9, 11), (8, 3, 64)]
+
+
+@pytest.mark.parametrize("dims", ND_NON_POW2, ids=lambda d: "x".join(map(str, d)))
+def test_nd_non_pow2_bluestein(gfw, oracle, dims):
+    # the reference exits on these sizes (src/fft2d.c:355-362); here every
+    # non-power-of-two axis runs Bluestein's chirp-z convolution
+    x = seeded(dims)
+    for d in (topefft.FORWARD, topefft.INVERSE):
+        got = run(gfw, dims, x, d)
+        assert rel_l2(got, oracle.dft(x, dims, d)) < TOL, (dims, d)
+
+
 ND_CASES = [(2, 2), (4, 4), (8, 4), (4, 16), (16, 16), (64, 32), (1, 8), (8, 1), (256, 256), (1024, 16),
             (16, 1024), (4096, 4), (4, 4096), (2, 2, 2), (4, 4, 4), (8, 16, 4), (16, 8, 32), (32, 32, 32),
             (1, 4, 8), (64, 4, 8), (4, 64, 8), (8, 4, 64)]
@@ -247,9 +261,9 @@ def test_profile_counters_and_passes():
 
 def test_error_paths():
     gfw = topefft.Framework()
-    x = np.zeros(12, complex)
+    x = np.zeros(3000 * 2, complex)
     with pytest.raises(topefft.TopeError) as e:
-        topefft.Plan(gfw, (12, 1), x)  # 2D/3D need powers of two
+        topefft.Plan(gfw, (3000, 2), x)  # 2D/3D non-powers of two up to 2048
     assert e.value.code == topefft.TOPE_ERROR_INVALID_SIZE
     p = topefft.Plan(gfw, (16,), np.zeros(16, complex))
     with pytest.raises(topefft.TopeError) as e:

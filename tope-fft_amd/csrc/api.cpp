@@ -67,6 +67,7 @@ struct Pass {
     int fixed_dir = -1;            // >= 0: the pass always runs in this direction (Bluestein's inner FFTs)
     int chirp_op = 0;              // PASS_CHIRP: CHIRP_PRE / CHIRP_MUL / CHIRP_POST
     long long chirp_n = 0, chirp_M = 0;
+    ChirpGeo chirp_g{1, 0, 1, 0, 1};
     const void *chirp_tab[2] = {nullptr, nullptr};  // transformed filters, forward / inverse
     const char *name = "";
 };
@@ -378,7 +379,8 @@ class Planner {
         a.pt_shift = pt_shift;
         a.scale = 1.0;
         a.xcd = (v->cfast && !std::getenv("TOPEFFT_NO_XCD_REMAP")) ? 1 : 0;
-        a.in_elems = a.out_elems = P->total;
+        // elements the buffers hold (Bluestein's padded rows can exceed the data)
+        a.in_elems = a.out_elems = std::max<long long>(P->total, (long long)(P->buf_bytes / P->elem));
         {
             // largest byte offset inside one block's footprint, in and out
             const long long span_in =
@@ -412,7 +414,7 @@ class Planner {
             b.pt_shift = pt_shift;
             b.scale = 1.0f;
             b.xcd = a.xcd;
-            b.in_elems = b.out_elems = P->total;
+            b.in_elems = b.out_elems = a.in_elems;
             b.pair = pair;
             b.dif_tw = static_cast<const cpx<float> *>(dif_tw);
         }
@@ -466,14 +468,25 @@ class Planner {
         return TOPE_SUCCESS;
     }
 
-    // Non-power-of-two length n by Bluestein's identity (kernels_chirp.hip):
-    // chirp pre-multiply into buffer 1 (length M, zero padded), forward
-    // FFT_M, product with the transformed filter of the direction, inverse
-    // FFT_M (both inner FFTs keep their direction whatever the outer one),
-    // chirp post-multiply into buffer 0.  The filters are transformed here,
-    // at plan time, by the same FFT_M passes.
-    int bluestein(long long n, long long M)
+    // Non-power-of-two axis of length n (lines g) by Bluestein's identity
+    // (kernels_chirp.hip): chirp pre-multiply of every line into its own
+    // zero-padded row of M elements in the other buffer, forward FFT_M over
+    // the rows, product with the transformed filter of the direction,
+    // inverse FFT_M (both inner FFTs keep their direction whatever the outer
+    // one), chirp post-multiply back into the line.  The data stays in
+    // buffer *cur.  The filters are transformed here, at plan time, by a
+    // one-row FFT_M.
+    static long long chirp_len(long long n)
     {
+        long long M = 1;
+        while (M < 2 * n - 1) M <<= 1;
+        return M;
+    }
+
+    int bluestein_axis(long long n, const Geo &g, int *cur)
+    {
+        const long long M = chirp_len(n), lines = g.O * g.C;
+        const int data = *cur, scratch = *cur ^ 1;
         const size_t tb = (size_t)M * 16;
         void *tab[2] = {nullptr, nullptr};
         for (int d = 0; d < 2; ++d) {
@@ -487,6 +500,7 @@ class Planner {
             ps.chirp_op = op;
             ps.chirp_n = n;
             ps.chirp_M = M;
+            ps.chirp_g = ChirpGeo{g.O, g.os, g.C, g.cs, g.is};
             ps.chirp_tab[0] = tab[0];
             ps.chirp_tab[1] = tab[1];
             ps.in_buf = bi;
@@ -494,30 +508,38 @@ class Planner {
             ps.name = op == CHIRP_PRE ? "chirp_pre" : op == CHIRP_MUL ? "chirp_mul" : "chirp_post";
             P->passes.push_back(ps);
         };
-        chirp_pass(CHIRP_PRE, 0, 1);
-        int cur = 1;
+        // the rows live in the scratch buffer; an inner FFT longer than one
+        // pass (1D only: one row) uses the data buffer, consumed by then
+        chirp_pass(CHIRP_PRE, data, scratch);
+        int rc_ = scratch;
         const size_t f0 = P->passes.size();
-        int rc = axis(M, Geo{1, 0, 1, M, 1}, &cur, 0);
+        int rc = axis(M, Geo{1, 0, lines, M, 1}, &rc_, -1);
         if (rc) return rc;
+        for (size_t i = f0; i < P->passes.size(); ++i) P->passes[i].fixed_dir = FORWARD;
+        chirp_pass(CHIRP_MUL, rc_, rc_);
         const size_t f1 = P->passes.size();
-        for (size_t i = f0; i < f1; ++i) P->passes[i].fixed_dir = FORWARD;
-        const int after_fwd = cur;
-        chirp_pass(CHIRP_MUL, cur, cur);
-        rc = axis(M, Geo{1, 0, 1, M, 1}, &cur, -1);
+        rc = axis(M, Geo{1, 0, lines, M, 1}, &rc_, -1);
         if (rc) return rc;
-        for (size_t i = f1 + 1; i < P->passes.size(); ++i) P->passes[i].fixed_dir = INVERSE;
-        chirp_pass(CHIRP_POST, cur, 0);
+        for (size_t i = f1; i < P->passes.size(); ++i) P->passes[i].fixed_dir = INVERSE;
+        chirp_pass(CHIRP_POST, rc_, data);
 
-        // transformed filters: b into buffer 1, the forward FFT_M passes,
-        // then the result (in buffer after_fwd) into the table
-        for (int d = 0; d < 2; ++d) {
-            hipError_t err = launch_chirp(CHIRP_FILTER, nullptr, P->buf[1], nullptr, n, M, 1.0, d, st);
-            for (size_t i = f0; i < f1 && err == hipSuccess; ++i)
+        // transformed filters: b into the scratch buffer, a one-row forward
+        // FFT_M (planned, run, then dropped from the pass list), copy out
+        const size_t g0 = P->passes.size();
+        int fc = scratch;
+        rc = axis(M, Geo{1, 0, 1, M, 1}, &fc, -1);
+        if (rc) return rc;
+        const size_t g1 = P->passes.size();
+        for (int d = 0; d < 2 && !rc; ++d) {
+            hipError_t err =
+                launch_chirp(CHIRP_FILTER, nullptr, P->buf[scratch], nullptr, ChirpGeo{1, 0, 1, 0, 1}, n, M, 1.0, d, st);
+            for (size_t i = g0; i < g1 && err == hipSuccess; ++i)
                 if (launch_pass(P, i, st, FORWARD)) err = hipErrorUnknown;
-            if (err == hipSuccess) err = hipMemcpyAsync(tab[d], P->buf[after_fwd], tb, hipMemcpyDeviceToDevice, st);
-            if (err != hipSuccess) return map_err(err);
+            if (err == hipSuccess) err = hipMemcpyAsync(tab[d], P->buf[fc], tb, hipMemcpyDeviceToDevice, st);
+            if (err != hipSuccess) rc = map_err(err);
         }
-        return TOPE_SUCCESS;
+        P->passes.resize(g0);
+        return rc;
     }
 
     // DFT-16 operand constants of the MFMA passes (one table per plan).
@@ -703,10 +725,12 @@ int launch_pass(PlanImpl *P, size_t i, hipStream_t s, int dir)
         hipError_t err = launch_mfma_rows(ps.mfma_mode, ps.mfma_n, ib, ob, m, sign, s);
         if (err != hipSuccess) return map_err(err);
     } else if (ps.kind == PASS_CHIRP) {
+        // 1/M undoes the unnormalised inner inverse FFT; the plan's 1/N of
+        // INVERSE rides on its last pass as for the axis passes
         const int inv = dir == INVERSE ? 1 : 0;
-        const double M = (double)ps.chirp_M, n = (double)ps.chirp_n;
-        const double post = inv ? 1.0 / (M * n) : 1.0 / M;
-        hipError_t err = launch_chirp(ps.chirp_op, ib, ob, ps.chirp_tab[inv], ps.chirp_n, ps.chirp_M, post, inv, s);
+        const double post = (1.0 / (double)ps.chirp_M) * (ps.last ? sc : 1.0);
+        hipError_t err = launch_chirp(ps.chirp_op, ib, ob, ps.chirp_tab[inv], ps.chirp_g, ps.chirp_n, ps.chirp_M,
+                                      post, inv, s);
         if (err != hipSuccess) return map_err(err);
     } else if (ps.kind == PASS_REAL) {
         const bool fwd = dir == FORWARD;
@@ -776,7 +800,8 @@ int build_plan(struct topeFFT *f, int shape, long long nx, long long ny, long lo
     if (nx < 1 || ny < 1 || nz < 1) return TOPE_ERROR_INVALID_SIZE;
     const long long total = nx * ny * nz;
     if (total > (1LL << 34)) return TOPE_ERROR_INVALID_SIZE;
-    if ((shape == 2 || shape == 3) && (!is_pow2(nx) || !is_pow2(ny) || !is_pow2(nz))) return TOPE_ERROR_INVALID_SIZE;
+    if ((shape == 2 || shape == 3) && prec != PREC_F64 && (!is_pow2(nx) || !is_pow2(ny) || !is_pow2(nz)))
+        return TOPE_ERROR_UNSUPPORTED;
     if (shape == SHAPE_BATCH && !is_pow2(nx)) return TOPE_ERROR_INVALID_SIZE;
     if (prec != PREC_F64 && prec != PREC_F32) return TOPE_ERROR_INVALID_ARG;
     if (mode != MP_VALU && (prec != PREC_F32 || (shape != SHAPE_BATCH && shape != 2 && shape != 3)))
@@ -799,20 +824,25 @@ int build_plan(struct topeFFT *f, int shape, long long nx, long long ny, long lo
     P->total = total;
     P->norm = shape == SHAPE_BATCH ? nx : total;
     P->bytes = (size_t)total * P->elem;
-    // non-power-of-two 1D above kChirpMin: Bluestein over the power of two
-    // M >= 2n - 1 (device arrays of M elements, transfers of n)
-    const bool chirpz = shape == 1 && !is_pow2(nx) && nx > kChirpMin;
-    long long chirp_M = 1;
-    if (chirpz) {
-        while (chirp_M < 2 * nx - 1) chirp_M <<= 1;
-        if (chirp_M > kMaxAxis) {
-            delete P;
-            return TOPE_ERROR_INVALID_SIZE;
+    // Non-power-of-two axes by Bluestein (1D above kChirpMin, every one in
+    // 2D/3D): the rows of M elements per line need a larger scratch buffer
+    bool chirpz = false;
+    size_t buf_elems = (size_t)total;
+    {
+        const long long dims3[3] = {nx, ny, nz};
+        for (int ax = 0; ax < 3; ++ax) {
+            const long long n = dims3[ax];
+            if (n == 1 || is_pow2(n) || (shape == 1 && n <= kChirpMin)) continue;
+            const long long M = Planner::chirp_len(n);
+            if (M > (shape == 1 ? kMaxAxis : (long long)kMaxFused)) {
+                delete P;
+                return TOPE_ERROR_INVALID_SIZE;
+            }
+            chirpz = true;
+            buf_elems = std::max(buf_elems, (size_t)(total / n * M));
         }
-        P->total = chirp_M;  // element count the inner passes address
-        P->norm = 1;         // scales applied by the chirp post-pass
     }
-    P->buf_bytes = chirpz ? (size_t)chirp_M * P->elem : P->bytes;
+    P->buf_bytes = buf_elems * P->elem;
     hipStream_t s = fw->stream;
 
     auto fail = [&](int rc) {
@@ -829,6 +859,11 @@ int build_plan(struct topeFFT *f, int shape, long long nx, long long ny, long lo
         err = hipMalloc(&P->buf[1], P->buf_bytes);
         if (err != hipSuccess) return fail(map_err(err));
     }
+    // one axis: the power-of-two passes, or Bluestein for the others
+    auto any_axis = [&](Planner &pl, long long n, const Geo &g, int *cur, int id) {
+        if (n > 1 && !is_pow2(n) && !(shape == 1 && n <= kChirpMin)) return pl.bluestein_axis(n, g, cur);
+        return pl.axis(n, g, cur, id);
+    };
 
     EventPair pre;
     err = time_start(pre, s);
@@ -865,7 +900,9 @@ int build_plan(struct topeFFT *f, int shape, long long nx, long long ny, long lo
     } else if (shape == SHAPE_BATCH) {
         rc = pl.axis(nx, Geo{1, 0, ny, nx, 1}, &cur, 0);
     } else if (chirpz) {
-        rc = pl.bluestein(nx, chirp_M);
+        rc = any_axis(pl, nx, Geo{1, 0, ny * nz, nx, 1}, &cur, 0);
+        if (!rc && shape >= 2) rc = any_axis(pl, ny, Geo{nz, nx * ny, nx, 1, nx}, &cur, 1);
+        if (!rc && shape >= 3) rc = any_axis(pl, nz, Geo{1, 0, nx * ny, 1, nx * ny}, &cur, 2);
     } else if (shape == 1 && !is_pow2(nx)) {
         // short non-power-of-two 1D: GPU DFT with exactly reduced phases
         void *roots = nullptr;

@@ -70,11 +70,16 @@ hipError_t launch_mfma_rows(int mode, int n, const void *in, void *out, const Mf
 // k = 0..m, n = 2m.
 hipError_t launch_real_pass(int dir, const void *in, void *out, const void *w, long long m, long long pitch,
                             long long rows, double s, hipStream_t st);
-// Bluestein chirp-z passes for non-power-of-two 1D (kernels_chirp.hip),
-// FP64: n the transform length, M the power-of-two convolution length.
+// Bluestein chirp-z passes for non-power-of-two axes (kernels_chirp.hip),
+// FP64: n the axis length, M the power-of-two convolution length; the lines
+// (o < O, c < C) of the axis sit at o os + c cs, elements is apart, and
+// line l is padded into row l of an M-element-row scratch array.
 enum { CHIRP_FILTER = 0, CHIRP_PRE = 1, CHIRP_MUL = 2, CHIRP_POST = 3 };
-hipError_t launch_chirp(int op, const void *in, void *out, const void *table, long long n, long long M, double s,
-                        int inverse, hipStream_t st);
+struct ChirpGeo {
+    long long O, os, C, cs, is;
+};
+hipError_t launch_chirp(int op, const void *in, void *out, const void *table, const ChirpGeo &g, long long n,
+                        long long M, double s, int inverse, hipStream_t st);
 // Mixed-precision column tiles (mfma_tile.hpp): 16 unit-stride lines per block.
 struct MfmaTileArgs;
 bool mfma_tile_supported(int n);
