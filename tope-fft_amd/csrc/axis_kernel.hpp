@@ -124,26 +124,111 @@ template <typename R, int N, int E, int TB, bool CFAST, bool SPLIT>
 struct Exchange {
     static constexpr int BYTES = lds_slots<N, TB>() * (SPLIT ? sizeof(R) : 2 * sizeof(R));
 };
-// Twiddles W^{k*m}, m = 1..r-1, of one butterfly.  Only the power-of-two
-// exponents m = 1, 2, 4, 8 are read from the table; every other one is the
-// product of two earlier entries (m = 2^a + b, b < 2^a), at most three
-// products deep.  This keeps 4 table loads in flight instead of r-1.
-template <typename R, int r>
-__device__ __forceinline__ void stage_twiddles(cpx<R> (&w)[r], const cpx<R> *__restrict__ t, int Ns)
+// Buffer resource over [base, base + bytes) (gfx9 raw buffer, no swizzle);
+// accesses past num_records read zero and drop stores.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, long long bytes)
 {
-    w[0] = cpx<R>{1, 0};
-#pragma unroll
-    for (int m = 1; m < r; m <<= 1) w[m] = t[(m - 1) * Ns];
-#pragma unroll
-    for (int m = 3; m < r; ++m) {
-        if ((m & (m - 1)) == 0) continue;
-        const int hi = 1 << (31 - __builtin_clz(m));
-        w[m] = cmul(w[hi], w[m - hi]);
+    const long long lim = 0x7fffffffLL;
+    const int nrec = (int)(bytes < lim ? (bytes > 0 ? bytes : 0) : lim);
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, nrec, 0x00020000);
+}
+
+typedef unsigned int tope_u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int tope_u32x2 __attribute__((ext_vector_type(2)));
+
+template <typename R>
+__device__ __forceinline__ cpx<R> buf_load(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    if constexpr (sizeof(R) == 8) {
+        tope_u32x4 u = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+        return __builtin_bit_cast(cpx<R>, u);
+    } else {
+        tope_u32x2 u = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+        return __builtin_bit_cast(cpx<R>, u);
     }
 }
 
-template <typename R, int N, int E, int SIGN, int S>
-__device__ __forceinline__ void run_stage(cpx<R> (&v)[E], int p, const cpx<R> *__restrict__ tw)
+// 128-bit stores take their whole offset in the VGPR and soffset = 0.  A
+// buffer store of more than 64 bits of data whose soffset is an SGPR is
+// exempt from the compiler's store-data hazard check (no wait state before
+// a VALU instruction overwrites the data registers), and on gfx950 such a
+// VALU write issued right after the store has replaced the upper dwords of
+// the stored value in some lanes.  With an inline-constant soffset the
+// compiler inserts the wait state.
+template <typename R>
+__device__ __forceinline__ void buf_store(cpx<R> v, __amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    if constexpr (sizeof(R) == 8)
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(tope_u32x4, v), r, voff + soff, 0, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(tope_u32x2, v), r, voff, soff, 0);
+}
+
+// Twiddles W^{k*m}, m = 1..r-1, of one butterfly: only the power-of-two
+// exponents m = 1, 2, 4, 8 are read; every other one is the product of two
+// earlier entries (m = 2^a + b, b < 2^a), at most three products deep, and
+// is applied as soon as it is formed (apply_twiddles) so that at most eight
+// twiddles are live beside the data.
+//
+// The global stage table has layout [(m-1)*Ns + k] from entry `base`; it is
+// read through a buffer resource so the lane part of the address is one
+// 32-bit offset (k) shared by every exponent and stage, the rest uniform.
+template <typename R, int r>
+__device__ __forceinline__ void load_twiddles(cpx<R> (&w)[r], __amdgpu_buffer_rsrc_t t, int kb, int base, int Ns)
+{
+    constexpr int CB = (int)sizeof(cpx<R>);
+#pragma unroll
+    for (int m = 1; m < r; m <<= 1) w[m] = buf_load<R>(t, kb, (base + (m - 1) * Ns) * CB);
+}
+
+template <int SIGN, typename R, int r>
+__device__ __forceinline__ void apply_twiddles(cpx<R> (&x)[r], cpx<R> (&w)[r])
+{
+#pragma unroll
+    for (int m = 1; m < r; ++m) {
+        if (m & (m - 1)) {
+            const int hi = 1 << (31 - __builtin_clz(m));
+            w[m] = cmul(w[hi], w[m - hi]);
+        }
+        x[m] = twmul<SIGN>(x[m], w[m]);
+    }
+}
+
+// LDS copy of the power-of-two rows of the stage table, for blocks that own
+// a whole CU: a table read from L2 waits behind the streaming HBM traffic,
+// an LDS read does not.  Stage s >= 1 holds log2(r) rows of Ns entries,
+// row b = exponent 2^b.
+template <int N, int E>
+struct TwLds {
+    using SC = Sched<N, E>;
+    static constexpr int count(int s) { return ilog2c(SC::radix(s)) * SC::span(s); }
+    static constexpr int off(int s)
+    {
+        int o = 0;
+        for (int i = 1; i < s; ++i) o += count(i);
+        return o;
+    }
+    static constexpr int TOTAL = off(SC::NST);
+    // global table index of LDS entry i
+    __device__ static int src(int i)
+    {
+        int s = 1;
+        while (s + 1 < SC::NST && i >= off(s + 1)) ++s;
+        const int j = i - off(s), ns = SC::span(s);
+        const int b = j / ns, k = j - b * ns;
+        return SC::twoff(s) + ((1 << b) - 1) * ns + k;
+    }
+};
+
+template <typename R, int r>
+__device__ __forceinline__ void load_twiddles_lds(cpx<R> (&w)[r], const cpx<R> *t, int Ns)
+{
+#pragma unroll
+    for (int b = 0; (1 << b) < r; ++b) w[1 << b] = t[b * Ns];
+}
+
+template <typename R, int N, int E, int SIGN, int S, bool TWL>
+__device__ __forceinline__ void run_stage(cpx<R> (&v)[E], int p, const cpx<R> *__restrict__ tw, const cpx<R> *twl)
 {
     using SC = Sched<N, E>;
     constexpr int r = SC::radix(S);
@@ -159,9 +244,11 @@ __device__ __forceinline__ void run_stage(cpx<R> (&v)[E], int p, const cpx<R> *_
             const int j = p + q * P;
             const int k = j & (Ns - 1);
             cpx<R> w[r];
-            stage_twiddles<R, r>(w, tw + SC::twoff(S) + k, Ns);
-#pragma unroll
-            for (int m = 1; m < r; ++m) x[m] = twmul<SIGN>(x[m], w[m]);
+            if constexpr (TWL) load_twiddles_lds<R, r>(w, twl + TwLds<N, E>::off(S) + k, Ns);
+            else
+                load_twiddles<R, r>(w, make_rsrc(tw, (long long)SC::TWSIZE * (int)sizeof(cpx<R>)),
+                                    k * (int)sizeof(cpx<R>), SC::twoff(S), Ns);
+            apply_twiddles<SIGN>(x, w);
         }
         dft<r, SIGN>(x);
 #pragma unroll
@@ -179,12 +266,12 @@ __device__ __forceinline__ int stage_dst(int j, int m)
     return (j / Ns) * Ns * r + (j & (Ns - 1)) + m * Ns;
 }
 
-template <typename R, int N, int E, int TB, bool CFAST, bool SPLIT, int SIGN, int S>
+template <typename R, int N, int E, int TB, bool CFAST, bool SPLIT, int SIGN, int S, bool TWL = false>
 __device__ __forceinline__ void run_stages(cpx<R> (&v)[E], int p, int t, const cpx<R> *__restrict__ tw,
-                                           void *lds_raw)
+                                           void *lds_raw, const cpx<R> *twl = nullptr)
 {
     using SC = Sched<N, E>;
-    run_stage<R, N, E, SIGN, S>(v, p, tw);
+    run_stage<R, N, E, SIGN, S, TWL>(v, p, tw, twl);
     if constexpr (S + 1 < SC::NST) {
         constexpr int r = SC::radix(S);
         constexpr int Q = E / r;
@@ -234,44 +321,32 @@ __device__ __forceinline__ void run_stages(cpx<R> (&v)[E], int p, int t, const c
                 for (int e = 0; e < E; ++e) v[e].y = lds[rbase + lds_off<TB, CFAST>(P * e)];
             }
         }
-        run_stages<R, N, E, TB, CFAST, SPLIT, SIGN, S + 1>(v, p, t, tw, lds_raw);
+        run_stages<R, N, E, TB, CFAST, SPLIT, SIGN, S + 1, TWL>(v, p, t, tw, lds_raw, twl);
     }
 }
 
-// Buffer resource over [base, base + bytes) (gfx9 raw buffer, no swizzle);
-// accesses past num_records read zero and drop stores.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, long long bytes)
+template <typename R, int N, int E, int TB, bool CFAST, bool SPLIT, bool DIF2>
+constexpr int axis_exchange_bytes()
 {
-    const long long lim = 0x7fffffffLL;
-    const int nrec = (int)(bytes < lim ? (bytes > 0 ? bytes : 0) : lim);
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, nrec, 0x00020000);
+    using SC = Sched<N, E>;
+    return (DIF2 || ((SC::P > 1) && (SC::NST > 1))) ? Exchange<R, N, E, TB, CFAST, SPLIT>::BYTES : 16;
+}
+// Stage twiddles go to LDS when the exchange alone already limits the block
+// to one per CU (more than half of the 160 KiB) and the table still fits.
+template <typename R, int N, int E, int TB, bool CFAST, bool SPLIT, bool DIF2>
+constexpr bool axis_tw_lds()
+{
+    constexpr int X = axis_exchange_bytes<R, N, E, TB, CFAST, SPLIT, DIF2>();
+    return Sched<N, E>::NST > 1 && X > 81920 && X + TwLds<N, E>::TOTAL * (int)sizeof(cpx<R>) <= 163840;
+}
+template <typename R, int N, int E, int TB, bool CFAST, bool SPLIT, bool DIF2>
+constexpr int axis_lds_bytes()
+{
+    return axis_exchange_bytes<R, N, E, TB, CFAST, SPLIT, DIF2>() +
+           (axis_tw_lds<R, N, E, TB, CFAST, SPLIT, DIF2>() ? TwLds<N, E>::TOTAL * (int)sizeof(cpx<R>) : 0);
 }
 
-typedef unsigned int tope_u32x4 __attribute__((ext_vector_type(4)));
-typedef unsigned int tope_u32x2 __attribute__((ext_vector_type(2)));
-
-template <typename R>
-__device__ __forceinline__ cpx<R> buf_load(__amdgpu_buffer_rsrc_t r, int voff, int soff)
-{
-    if constexpr (sizeof(R) == 8) {
-        tope_u32x4 u = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
-        return __builtin_bit_cast(cpx<R>, u);
-    } else {
-        tope_u32x2 u = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
-        return __builtin_bit_cast(cpx<R>, u);
-    }
-}
-
-template <typename R>
-__device__ __forceinline__ void buf_store(cpx<R> v, __amdgpu_buffer_rsrc_t r, int voff, int soff)
-{
-    if constexpr (sizeof(R) == 8)
-        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(tope_u32x4, v), r, voff, soff, 0);
-    else
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(tope_u32x2, v), r, voff, soff, 0);
-}
-
-// One launch = one axis pass over TB transforms per block.  POST applies
+// One logical block (tile blk) of an axis pass: TB transforms.  POST applies
 // the four-step post-twiddle; the output scale is runtime-uniform.
 //
 // Global accesses go through buffer resources based at the block's first
@@ -279,22 +354,26 @@ __device__ __forceinline__ void buf_store(cpx<R> v, __amdgpu_buffer_rsrc_t r, in
 // stride e*P*is*16 is a uniform (SGPR) offset, which keeps address
 // registers out of the way of the 4*E data registers.  Blocks whose
 // footprint exceeds 2^31 bytes run the BIG instantiation (64-bit pointers).
-template <typename R, int N, int E, int TB, bool CFAST, bool SPLIT, int SIGN, bool POST, bool BIG, bool DIF2 = false>
-__global__ void __launch_bounds__(TB *(N / E))
-axis_kernel(const cpx<R> *in, cpx<R> *out, const AxisArgs<R> a)
+template <typename R, int N, int E, int TB, bool CFAST, bool SPLIT, int SIGN, bool POST, bool BIG, bool DIF2>
+__device__ __forceinline__ void axis_block(const cpx<R> *in, cpx<R> *out, const AxisArgs<R> &a, long long blk,
+                                           unsigned char *lds)
 {
     using SC = Sched<N, E>;
     constexpr int P = SC::P;
     static_assert(!DIF2 || (!CFAST && TB % 2 == 0), "DIF2 pairs rows within a row block");
-    constexpr bool USE_LDS = DIF2 || ((P > 1) && (SC::NST > 1));
-    constexpr int LDS_BYTES = USE_LDS ? Exchange<R, N, E, TB, CFAST, SPLIT>::BYTES : 16;
     constexpr int CB = (int)sizeof(cpx<R>);
-    __shared__ __align__(16) unsigned char lds[LDS_BYTES];
+    constexpr bool TWL = axis_tw_lds<R, N, E, TB, CFAST, SPLIT, DIF2>();
+    constexpr int XB = axis_exchange_bytes<R, N, E, TB, CFAST, SPLIT, DIF2>();
 
     const int tid = threadIdx.x;
     const int t = CFAST ? (tid % TB) : (tid / P);
     const int p = CFAST ? (tid / TB) : (tid % P);
-    const long long blk = xcd_block(a.xcd);
+    // LDS twiddle rows, loaded ahead of the data; stage 1 reads them after
+    // the first exchange's barriers
+    cpx<R> *twl = reinterpret_cast<cpx<R> *>(lds + XB);
+    if constexpr (TWL) {
+        for (int i = tid; i < TwLds<N, E>::TOTAL; i += TB * P) twl[i] = a.tw[TwLds<N, E>::src(i)];
+    }
     const long long o = blk / a.tiles;
     const int tile = (int)(blk - o * a.tiles);
     // DIF2: TB/2 row pairs per block, transform t = 2q + h
@@ -329,7 +408,11 @@ axis_kernel(const cpx<R> *in, cpx<R> *out, const AxisArgs<R> a)
         }
     }
 
-    run_stages<R, N, E, TB, CFAST, SPLIT, SIGN, 0>(v, p, t, a.tw, lds);
+    if constexpr (TWL) {
+        run_stages<R, N, E, TB, CFAST, SPLIT, SIGN, 0, true>(v, p, t, a.tw, lds, twl);
+    } else {
+        run_stages<R, N, E, TB, CFAST, SPLIT, SIGN, 0>(v, p, t, a.tw, lds);
+    }
 
     if constexpr (DIF2) {
         // radix-2 DIF step of the next axis between the rows of each pair,
@@ -369,8 +452,10 @@ axis_kernel(const cpx<R> *in, cpx<R> *out, const AxisArgs<R> a)
             v[e] = twmul<SIGN>(v[e], w);
         }
     }
+    if (a.scale != R(1)) {  // uniform: only the last pass of an inverse scales
 #pragma unroll
-    for (int e = 0; e < E; ++e) v[e] = scale(v[e], a.scale);
+        for (int e = 0; e < E; ++e) v[e] = scale(v[e], a.scale);
+    }
     if constexpr (!BIG) {
         const long long obase = o * a.out_os + (long long)c0 * a.out_cs;
         const __amdgpu_buffer_rsrc_t rout = make_rsrc(out + obase, (a.out_elems - obase) * CB);
@@ -383,6 +468,15 @@ axis_kernel(const cpx<R> *in, cpx<R> *out, const AxisArgs<R> a)
 #pragma unroll
         for (int e = 0; e < E; ++e) dst[(long long)(p + P * e) * a.out_is] = v[e];
     }
+}
+
+// One launch = one axis pass, one workgroup per logical block.
+template <typename R, int N, int E, int TB, bool CFAST, bool SPLIT, int SIGN, bool POST, bool BIG, bool DIF2 = false>
+__global__ void __launch_bounds__(TB *(N / E))
+axis_kernel(const cpx<R> *in, cpx<R> *out, const AxisArgs<R> a)
+{
+    __shared__ __align__(16) unsigned char lds[axis_lds_bytes<R, N, E, TB, CFAST, SPLIT, DIF2>()];
+    axis_block<R, N, E, TB, CFAST, SPLIT, SIGN, POST, BIG, DIF2>(in, out, a, xcd_block(a.xcd), lds);
 }
 
 }  // namespace tope

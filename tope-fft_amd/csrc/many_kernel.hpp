@@ -45,7 +45,12 @@ __device__ __forceinline__ void many_rows(const cpx<R> *in, cpx<R> *out, const M
     constexpr int TB = MANY_THREADS / P;
     using SC = Sched<N, E>;
     constexpr int CB = (int)sizeof(cpx<R>);
-    const int tid = threadIdx.x;
+    // Opaque to the optimiser: nothing lane-dependent (LDS bases, twiddle
+    // and data offsets) is hoisted out of the task loop, where it would stay
+    // live across every class and spill under the 128-VGPR cap of a
+    // 1024-thread block; it is recomputed per task instead (a few VALU ops).
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
     const int t = tid / P, p = tid % P;
     const bool valid = t < tk.count;
     const long long off = valid ? a.offs[a.order[tk.first + t]] : 0;
