@@ -19,9 +19,11 @@
  *     unallocated scratch buffer on its radix path, src/fft1d.c:140-142).
  *   - 1D sizes that are not powers of two are transformed in both
  *     directions (the reference's DFT kernel ignores `dir`,
- *     src/kernels1D.cl:549-573): a GPU DFT up to 64, Bluestein's chirp-z
- *     convolution above; 2D/3D axes that are not powers of two (up to 2048)
- *     run Bluestein where the reference exits (src/fft2d.c:355-362).
+ *     src/kernels1D.cl:549-573): 7-smooth lengths (factors 2, 3, 5, 7) up
+ *     to 4096 by a mixed-radix pass, other lengths by a GPU DFT up to 64
+ *     and Bluestein's chirp-z convolution above; 2D/3D axes that are not
+ *     powers of two (7-smooth up to 4096, others up to 2048) run the same
+ *     way where the reference exits (src/fft2d.c:355-362).
  *   - the n == 2 1D inverse divides both real and imaginary parts by 2
  *     (src/fft1d.c:23-26 divides only the real parts).
  *   - tope*Destroy frees the plan; the framework (stream) is released when

@@ -71,6 +71,70 @@ def test_1d_large_four_step_bins(gfw, oracle):
     assert rel_l2(back, x) < TOL
 
 
+def smooth7(n):
+    for p in (2, 3, 5, 7):
+        while n % p == 0:
+            n //= p
+    return n == 1
+
+
+MIXED_1D = [6, 12, 15, 49, 96, 100, 243, 343, 625, 1000, 2187, 2401, 3000, 3125, 3969, 4000]
+
+
+@pytest.mark.parametrize("n", MIXED_1D)
+def test_1d_mixed_radix(oracle, n):
+    # 7-smooth lengths up to 4096: one in-place mixed-radix pass (radix
+    # 16/8/4/2, 7, 5, 3) instead of the reference's O(n^2) DFT
+    x = seeded((n,))
+    fw = topefft.Framework()
+    p = topefft.Plan(fw, (n,), x)
+    names = p.pass_names()
+    fwd = p.exec(topefft.FORWARD)
+    back = p.exec(topefft.INVERSE)
+    p.destroy()
+    fw.destroy()
+    assert names == ["mixed_radix_f64"], names
+    assert rel_l2(fwd, oracle.dft(x, (n,), topefft.FORWARD)) < TOL, n
+    assert rel_l2(back, x) < TOL, n
+    assert rel_l2(topefft.transform(x, (n,), topefft.INVERSE), oracle.dft(x, (n,), topefft.INVERSE)) < TOL
+
+
+@pytest.mark.parametrize("n", [105, 1000, 2187])
+def test_1d_mixed_radix_closed_forms(n):
+    # delta -> all ones; exp(2 pi i k0 j / n) -> n at k0 only
+    d = np.zeros(n, complex)
+    d[0] = 1.0
+    assert np.max(np.abs(topefft.transform(d, (n,), topefft.FORWARD) - 1.0)) < 1e-13
+    k0 = 7
+    e = np.exp(2j * np.pi * k0 * np.arange(n) / n)
+    want = np.zeros(n, complex)
+    want[k0] = n
+    assert np.max(np.abs(topefft.transform(e, (n,), topefft.FORWARD) - want)) < 1e-9 * n
+
+
+MIXED_ND = [(48, 36), (100, 60), (1000, 7), (3, 729), (1024, 1000), (4000, 3), (12, 10, 14), (60, 9, 25),
+            (7, 256, 5), (100, 11)]
+
+
+@pytest.mark.parametrize("dims", MIXED_ND, ids=lambda d: "x".join(map(str, d)))
+def test_nd_mixed_radix(oracle, dims):
+    # every 7-smooth non-power-of-two axis (rows, columns, the 3D Z axis)
+    # runs the mixed-radix pass; 11 (not 7-smooth) runs Bluestein beside it
+    x = seeded(dims)
+    fw = topefft.Framework()
+    p = topefft.Plan(fw, dims, x)
+    names = p.pass_names()
+    fwd = p.exec(topefft.FORWARD)
+    back = p.exec(topefft.INVERSE)
+    p.destroy()
+    fw.destroy()
+    mixed_axes = sum(1 for n in dims if n & (n - 1) and smooth7(n))
+    assert names.count("mixed_radix_f64") == mixed_axes, names
+    assert ("chirp_pre" in names) == (11 in dims), names
+    assert rel_l2(fwd, oracle.dft(x, dims, topefft.FORWARD)) < TOL, dims
+    assert rel_l2(back, x) < TOL, dims
+
+
 @pytest.mark.parametrize("n", [3, 5, 6, 7, 12, 17, 100, 243, 1000])
 def test_1d_non_pow2_dft(gfw, oracle, n):
     x = seeded((n,))
@@ -111,9 +175,10 @@ ND_NON_POW2 = [(12, 1), (3, 5), (100, 64), (64, 100), (30, 50), (2048 - 1, 4), (
 
 
 @pytest.mark.parametrize("dims", ND_NON_POW2, ids=lambda d: "x".join(map(str, d)))
-def test_nd_non_pow2_bluestein(gfw, oracle, dims):
-    # the reference exits on these sizes (src/fft2d.c:355-362); here every
-    # non-power-of-two axis runs Bluestein's chirp-z convolution
+def test_nd_non_pow2(gfw, oracle, dims):
+    # the reference exits on these sizes (src/fft2d.c:355-362); here 7-smooth
+    # axes run the mixed-radix pass, the others Bluestein's chirp-z
+    # convolution (2047, 11)
     x = seeded(dims)
     for d in (topefft.FORWARD, topefft.INVERSE):
         got = run(gfw, dims, x, d)
@@ -261,10 +326,12 @@ def test_profile_counters_and_passes():
 
 def test_error_paths():
     gfw = topefft.Framework()
-    x = np.zeros(3000 * 2, complex)
-    with pytest.raises(topefft.TopeError) as e:
-        topefft.Plan(gfw, (3000, 2), x)  # 2D/3D non-powers of two up to 2048
-    assert e.value.code == topefft.TOPE_ERROR_INVALID_SIZE
+    # 2D/3D axes that are not powers of two: 7-smooth up to 4096 (mixed
+    # radix), others up to 2048 (Bluestein)
+    for n in (3001, 5000):
+        with pytest.raises(topefft.TopeError) as e:
+            topefft.Plan(gfw, (n, 2), np.zeros(n * 2, complex))
+        assert e.value.code == topefft.TOPE_ERROR_INVALID_SIZE, n
     p = topefft.Plan(gfw, (16,), np.zeros(16, complex))
     with pytest.raises(topefft.TopeError) as e:
         p.exec(5)

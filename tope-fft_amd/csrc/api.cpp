@@ -46,7 +46,15 @@ struct FrameworkImpl {
     EventPair last;  // last exec's device time (profileThis)
 };
 
-enum PassKind { PASS_AXIS = 0, PASS_DFT = 1, PASS_MFMA = 2, PASS_MFMA_TILE = 3, PASS_REAL = 4, PASS_CHIRP = 5 };
+enum PassKind {
+    PASS_AXIS = 0,
+    PASS_DFT = 1,
+    PASS_MFMA = 2,
+    PASS_MFMA_TILE = 3,
+    PASS_REAL = 4,
+    PASS_CHIRP = 5,
+    PASS_MIXED = 6
+};
 
 struct Pass {
     int kind = PASS_AXIS;
@@ -69,6 +77,7 @@ struct Pass {
     long long chirp_n = 0, chirp_M = 0;
     ChirpGeo chirp_g{1, 0, 1, 0, 1};
     const void *chirp_tab[2] = {nullptr, nullptr};  // transformed filters, forward / inverse
+    MixedArgs mx{};                                 // PASS_MIXED
     const char *name = "";
 };
 
@@ -476,6 +485,30 @@ class Planner {
     // one), chirp post-multiply back into the line.  The data stays in
     // buffer *cur.  The filters are transformed here, at plan time, by a
     // one-row FFT_M.
+    // Axis of 7-smooth length n <= 4096 that is not a power of two: one
+    // in-place mixed-radix pass (kernels_mixed.hip), twiddles and radix-3/5/7
+    // constants from a table of W_n^i.
+    int mixed_axis(long long n, const Geo &g, int cur, int axis_id)
+    {
+        void *roots = nullptr;
+        hipError_t err = power_table(n, 1, n, &roots);
+        if (err != hipSuccess) return map_err(err);
+        Pass ps;
+        ps.kind = PASS_MIXED;
+        ps.mx = mixed_plan(n, g.O, g.os, g.C, g.cs, g.is);
+        ps.mx.roots = static_cast<const cpx<double> *>(roots);
+        ps.grid = mixed_grid(ps.mx);
+        ps.in_buf = ps.out_buf = cur;
+        ps.name = "mixed_radix_f64";
+        P->passes.push_back(ps);
+        if (axis_id >= 0 && !P->axis_tw[axis_id].table) {
+            P->axis_tw[axis_id].n = (int)n;
+            P->axis_tw[axis_id].e = 0;
+            P->axis_tw[axis_id].table = roots;
+        }
+        return TOPE_SUCCESS;
+    }
+
     static long long chirp_len(long long n)
     {
         long long M = 1;
@@ -724,6 +757,11 @@ int launch_pass(PlanImpl *P, size_t i, hipStream_t s, int dir)
         m.scale = ps.last ? (float)sc : 1.0f;  // 1/N is a power of two: exact in FP32
         hipError_t err = launch_mfma_rows(ps.mfma_mode, ps.mfma_n, ib, ob, m, sign, s);
         if (err != hipSuccess) return map_err(err);
+    } else if (ps.kind == PASS_MIXED) {
+        MixedArgs m = ps.mx;
+        m.scale = ps.last ? sc : 1.0;
+        hipError_t err = launch_mixed(ib, ob, m, sign, s);
+        if (err != hipSuccess) return map_err(err);
     } else if (ps.kind == PASS_CHIRP) {
         // 1/M undoes the unnormalised inner inverse FFT; the plan's 1/N of
         // INVERSE rides on its last pass as for the axis passes
@@ -830,9 +868,9 @@ int build_plan(struct topeFFT *f, int shape, long long nx, long long ny, long lo
     size_t buf_elems = (size_t)total;
     {
         const long long dims3[3] = {nx, ny, nz};
-        for (int ax = 0; ax < 3; ++ax) {
+        for (int ax = 0; ax < (shape == SHAPE_BATCH ? 1 : 3); ++ax) {
             const long long n = dims3[ax];
-            if (n == 1 || is_pow2(n) || (shape == 1 && n <= kChirpMin)) continue;
+            if (n == 1 || is_pow2(n) || mixed_supported(n) || (shape == 1 && n <= kChirpMin)) continue;
             const long long M = Planner::chirp_len(n);
             if (M > (shape == 1 ? kMaxAxis : (long long)kMaxFused)) {
                 delete P;
@@ -859,11 +897,16 @@ int build_plan(struct topeFFT *f, int shape, long long nx, long long ny, long lo
         err = hipMalloc(&P->buf[1], P->buf_bytes);
         if (err != hipSuccess) return fail(map_err(err));
     }
-    // one axis: the power-of-two passes, or Bluestein for the others
+    // one axis: the power-of-two passes; 7-smooth lengths up to 4096 by the
+    // mixed-radix pass; Bluestein for the others
     auto any_axis = [&](Planner &pl, long long n, const Geo &g, int *cur, int id) {
-        if (n > 1 && !is_pow2(n) && !(shape == 1 && n <= kChirpMin)) return pl.bluestein_axis(n, g, cur);
+        if (n > 1 && !is_pow2(n)) {
+            if (mixed_supported(n)) return pl.mixed_axis(n, g, *cur, id);
+            return pl.bluestein_axis(n, g, cur);
+        }
         return pl.axis(n, g, cur, id);
     };
+    const bool nonpow2 = !is_pow2(nx) || (shape >= 2 && !is_pow2(ny)) || (shape >= 3 && !is_pow2(nz));
 
     EventPair pre;
     err = time_start(pre, s);
@@ -899,7 +942,7 @@ int build_plan(struct topeFFT *f, int shape, long long nx, long long ny, long lo
         P->axis_tw[0].table = tw;
     } else if (shape == SHAPE_BATCH) {
         rc = pl.axis(nx, Geo{1, 0, ny, nx, 1}, &cur, 0);
-    } else if (chirpz) {
+    } else if (nonpow2 && !(shape == 1 && !chirpz && !mixed_supported(nx))) {
         rc = any_axis(pl, nx, Geo{1, 0, ny * nz, nx, 1}, &cur, 0);
         if (!rc && shape >= 2) rc = any_axis(pl, ny, Geo{nz, nx * ny, nx, 1, nx}, &cur, 1);
         if (!rc && shape >= 3) rc = any_axis(pl, nz, Geo{1, 0, nx * ny, 1, nx * ny}, &cur, 2);

@@ -75,6 +75,22 @@ hipError_t launch_real_pass(int dir, const void *in, void *out, const void *w, l
 // (o < O, c < C) of the axis sit at o os + c cs, elements is apart, and
 // line l is padded into row l of an M-element-row scratch array.
 enum { CHIRP_FILTER = 0, CHIRP_PRE = 1, CHIRP_MUL = 2, CHIRP_POST = 3 };
+// Mixed-radix (2, 3, 4, 5, 7, 8, 16) in-place axis pass for 7-smooth
+// lengths up to MIXED_MAX_N (kernels_mixed.hip), FP64: line l = o C + c
+// (o < O, c < C) starts at o os + c cs, its elements are `is` apart; roots
+// holds W_n^i, i < n (forward sign).
+constexpr int MIXED_MAX_N = 4096;
+struct MixedArgs {
+    long long O, os, C, cs, is, lines;
+    int n, tb, nst;
+    int radix[16];
+    const cpx<double> *roots;
+    double scale;
+};
+bool mixed_supported(long long n);
+MixedArgs mixed_plan(long long n, long long O, long long os, long long C, long long cs, long long is);
+unsigned mixed_grid(const MixedArgs &a);
+hipError_t launch_mixed(const void *in, void *out, const MixedArgs &a, int sign, hipStream_t st);
 struct ChirpGeo {
     long long O, os, C, cs, is;
 };

@@ -1,0 +1,211 @@
+// Mixed-radix axis pass for lengths n = 2^a 3^b 5^c 7^d <= 4096 (FP64).
+// The reference runs such 1D sizes as an O(n^2) DFT (src/fft1d.c:148-232,
+// src/kernels1D.cl:549-573) and rejects them in 2D/3D (src/fft2d.c:355-362,
+// src/fft3d.c:522-533).
+//
+// Self-sorting Stockham, decimation in time, with a runtime radix list
+// (radix 16/8/4/2 for the power-of-two part, then 7, 5, 3): stage s of
+// radix r at span Ns treats butterfly j < n/r, reads j + m n/r (m < r),
+// multiplies element m by W_{Ns r}^{(j mod Ns) m} and writes output m of its
+// r-point DFT to (j / Ns) Ns r + (j mod Ns) + m Ns; starting at Ns = 1 the
+// result is in natural order.  A block owns whole lines: it loads them into
+// LDS, runs every stage between two LDS buffers (a barrier per stage), and
+// writes them back, so the pass is in place.  Every twiddle and every
+// radix-3/5/7 constant is an entry of the plan's table W_n^i (exactly
+// reduced phases, forward sign), so no device sin/cos is evaluated.
+#include <algorithm>
+
+#include "internal.hpp"
+
+namespace tope {
+
+
+template <int SIGN>
+__device__ __forceinline__ cpx<double> root(const cpx<double> *__restrict__ w, long long i)
+{
+    const cpx<double> v = w[i];
+    return SIGN < 0 ? v : cpx<double>{v.x, -v.y};
+}
+
+// Odd prime radix r by the conjugate-pair form: with a_j = x_j + x_{r-j},
+// b_j = x_j - x_{r-j} and W_r^{jk} = c + i s,
+//   y_k = x_0 + sum_{j=1}^{(r-1)/2} (a_j c + i b_j s).
+template <int r, int SIGN>
+__device__ __forceinline__ void dft_odd(cpx<double> (&x)[r], const cpx<double> *__restrict__ w, long long step)
+{
+    constexpr int h = (r - 1) / 2;
+    cpx<double> a[h], b[h], wr[r];
+#pragma unroll
+    for (int j = 1; j <= h; ++j) {
+        a[j - 1] = x[j] + x[r - j];
+        b[j - 1] = x[j] - x[r - j];
+    }
+#pragma unroll
+    for (int m = 1; m < r; ++m) wr[m] = root<SIGN>(w, m * step);
+    cpx<double> y[r];
+    y[0] = x[0];
+#pragma unroll
+    for (int j = 0; j < h; ++j) y[0] = y[0] + a[j];
+#pragma unroll
+    for (int k = 1; k < r; ++k) {
+        cpx<double> acc = x[0];
+#pragma unroll
+        for (int j = 1; j <= h; ++j) {
+            const cpx<double> c = wr[(j * k) % r];
+            acc.x += a[j - 1].x * c.x - b[j - 1].y * c.y;
+            acc.y += a[j - 1].y * c.x + b[j - 1].x * c.y;
+        }
+        y[k] = acc;
+    }
+#pragma unroll
+    for (int k = 0; k < r; ++k) x[k] = y[k];
+}
+
+// One stage of compile-time radix r over the block's lines, LDS to LDS.
+template <int r, int SIGN, int NT>
+__device__ __forceinline__ void mixed_stage(const cpx<double> *src, cpx<double> *dst, int n, int tb, int Ns,
+                                            const cpx<double> *__restrict__ roots)
+{
+    const int nb = n / r;  // butterflies per line
+    const long long tws = (long long)n / ((long long)Ns * r);
+    for (int bi = threadIdx.x; bi < tb * nb; bi += NT) {
+        const int t = bi / nb, j = bi - t * nb;
+        const int kk = j % Ns;
+        cpx<double> x[r];
+        const cpx<double> *sl = src + t * n + j;
+#pragma unroll
+        for (int m = 0; m < r; ++m) x[m] = sl[m * nb];
+        if (Ns > 1) {
+#pragma unroll
+            for (int m = 1; m < r; ++m) x[m] = cmul(x[m], root<SIGN>(roots, (long long)kk * m * tws));
+        }
+        if constexpr (r == 3 || r == 5 || r == 7) dft_odd<r, SIGN>(x, roots, n / r);
+        else dft<r, SIGN>(x);
+        cpx<double> *dl = dst + t * n + (j / Ns) * Ns * r + kk;
+#pragma unroll
+        for (int m = 0; m < r; ++m) dl[m * Ns] = x[m];
+    }
+}
+
+// CAP: elements of one LDS buffer (two buffers: 2 * CAP * 16 bytes); NT
+// threads (the single-line 4096-element blocks own a CU: more waves to
+// cover the twiddle-load latency).
+template <int SIGN, int CAP, int NT>
+__global__ void __launch_bounds__(NT) mixed_kernel(const cpx<double> *in, cpx<double> *out, const MixedArgs a)
+{
+    __shared__ cpx<double> lds[2 * CAP];
+    const int n = a.n, tb = a.tb;
+    const int span_elems = tb * n;
+    const bool cfast = a.is != 1;  // strided axis: neighbouring lanes take neighbouring lines
+    // strided lines share 128-byte segments with the neighbouring blocks'
+    // lines: keep neighbours on one XCD (one L2)
+    const long long l0 = xcd_block(cfast ? 1 : 0) * tb;
+
+    // element idx of the block -> (line t, index k), and its global offset
+    auto split = [&](int idx, int &t, int &k) {
+        if (cfast) {
+            t = idx % tb;
+            k = idx / tb;
+        } else {
+            t = idx / n;
+            k = idx % n;
+        }
+    };
+    auto gaddr = [&](long long l, int k) { return (l / a.C) * a.os + (l % a.C) * a.cs + (long long)k * a.is; };
+
+    for (int idx = threadIdx.x; idx < span_elems; idx += NT) {
+        int t, k;
+        split(idx, t, k);
+        const long long l = l0 + t;
+        lds[t * n + k] = l < a.lines ? in[gaddr(l, k)] : cpx<double>{0, 0};
+    }
+    __syncthreads();
+
+    cpx<double> *src = lds, *dst = lds + CAP;
+    int Ns = 1;
+    for (int s = 0; s < a.nst; ++s) {
+        const int r = a.radix[s];
+        switch (r) {
+        case 2: mixed_stage<2, SIGN, NT>(src, dst, n, tb, Ns, a.roots); break;
+        case 3: mixed_stage<3, SIGN, NT>(src, dst, n, tb, Ns, a.roots); break;
+        case 4: mixed_stage<4, SIGN, NT>(src, dst, n, tb, Ns, a.roots); break;
+        case 5: mixed_stage<5, SIGN, NT>(src, dst, n, tb, Ns, a.roots); break;
+        case 7: mixed_stage<7, SIGN, NT>(src, dst, n, tb, Ns, a.roots); break;
+        case 8: mixed_stage<8, SIGN, NT>(src, dst, n, tb, Ns, a.roots); break;
+        default: mixed_stage<16, SIGN, NT>(src, dst, n, tb, Ns, a.roots); break;
+        }
+        __syncthreads();
+        cpx<double> *tmp = src;
+        src = dst;
+        dst = tmp;
+        Ns *= r;
+    }
+
+    for (int idx = threadIdx.x; idx < span_elems; idx += NT) {
+        int t, k;
+        split(idx, t, k);
+        const long long l = l0 + t;
+        if (l < a.lines) {
+            const cpx<double> v = src[t * n + k];
+            out[gaddr(l, k)] = cpx<double>{v.x * a.scale, v.y * a.scale};
+        }
+    }
+}
+
+bool mixed_supported(long long n)
+{
+    if (n < 2 || n > MIXED_MAX_N) return false;
+    for (long long p : {2, 3, 5, 7})
+        while (n % p == 0) n /= p;
+    return n == 1;
+}
+
+MixedArgs mixed_plan(long long n, long long O, long long os, long long C, long long cs, long long is)
+{
+    MixedArgs a{};
+    a.n = (int)n;
+    a.O = O;
+    a.os = os;
+    a.C = C;
+    a.cs = cs;
+    a.is = is;
+    a.lines = O * C;
+    long long m = n;
+    int ns = 0;
+    while (m % 16 == 0) a.radix[ns++] = 16, m /= 16;
+    if (m % 8 == 0) a.radix[ns++] = 8, m /= 8;
+    if (m % 4 == 0) a.radix[ns++] = 4, m /= 4;
+    if (m % 2 == 0) a.radix[ns++] = 2, m /= 2;
+    for (int p : {7, 5, 3})
+        while (m % p == 0) a.radix[ns++] = p, m /= p;
+    a.nst = ns;
+    // lines per block: as many as fit 2048 elements (two 32 KiB LDS buffers,
+    // two blocks per CU), one line of up to 4096 otherwise
+    long long tb = std::max<long long>(1, 2048 / n);
+    if (tb > a.lines) tb = a.lines;
+    a.tb = (int)tb;
+    a.scale = 1.0;
+    return a;
+}
+
+unsigned mixed_grid(const MixedArgs &a) { return (unsigned)((a.lines + a.tb - 1) / a.tb); }
+
+hipError_t launch_mixed(const void *in, void *out, const MixedArgs &a, int sign, hipStream_t st)
+{
+    const long long grid = (a.lines + a.tb - 1) / a.tb;
+    if (grid > 0x7fffffffLL || a.nst < 1 || a.nst > 16 || a.tb < 1 || (long long)a.tb * a.n > MIXED_MAX_N)
+        return hipErrorInvalidValue;
+    const cpx<double> *i = static_cast<const cpx<double> *>(in);
+    cpx<double> *o = static_cast<cpx<double> *>(out);
+    const dim3 g((unsigned)grid);
+    if ((long long)a.tb * a.n <= 2048) {
+        if (sign < 0) hipLaunchKernelGGL((mixed_kernel<-1, 2048, 256>), g, dim3(256), 0, st, i, o, a);
+        else hipLaunchKernelGGL((mixed_kernel<1, 2048, 256>), g, dim3(256), 0, st, i, o, a);
+    } else {
+        if (sign < 0) hipLaunchKernelGGL((mixed_kernel<-1, MIXED_MAX_N, 1024>), g, dim3(1024), 0, st, i, o, a);
+        else hipLaunchKernelGGL((mixed_kernel<1, MIXED_MAX_N, 1024>), g, dim3(1024), 0, st, i, o, a);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace tope
