@@ -231,6 +231,23 @@ def small_1d_leg(fw, steps, warmup):
             "passes_us": [{"kernel": nm, "us": round(float(m) * 1e3, 2)} for nm, m in zip(names, pass_ms)]}
 
 
+def vendor_leg(steps):
+    """rocFFT (ROCm's FFT library, run-only comparator) on the same
+    device-resident transforms, timed by tope-fft_amd/lib/vendor_compare
+    with HIP events on one stream beside libtopefft; also the rel-L2 between
+    the two libraries' outputs."""
+    import json
+    import subprocess
+    exe = os.path.join(REPO, "tope-fft_amd", "lib", "vendor_compare")
+    if not os.path.exists(exe):
+        return {"skipped": "vendor_compare not built"}
+    r = subprocess.run([exe, str(steps), "2d4096", "3d256", "1d65536", "batch64", "batch32"], capture_output=True,
+                       text=True, timeout=600)
+    if r.returncode != 0:
+        return {"error": (r.stderr or r.stdout)[-300:]}
+    return [json.loads(line) for line in r.stdout.splitlines() if line.startswith("{")]
+
+
 def cpu_baseline(x, dims, budget_s=12.0):
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle_py
@@ -281,7 +298,7 @@ def main():
     dom_bytes = 2 * 16 * n2  # one read + one write of the array per pass
     achieved = gbs(dom_bytes, pass2[dom] * 1e-3)
 
-    legs = set(args.legs.split(",")) if args.legs != "all" else {"1d", "3d256", "mp", "many", "3d512", "r2c"}
+    legs = set(args.legs.split(",")) if args.legs != "all" else {"1d", "3d256", "mp", "many", "3d512", "r2c", "vendor"}
     extra = {}
     if "1d" in legs:
         extra["1d_65536"] = small_1d_leg(fw, args.steps, args.warmup)
@@ -307,6 +324,8 @@ def main():
         extra["r2c_2d_4096"] = real_2d_leg(fw, args.steps, args.warmup)
     if "3d512" in legs:
         extra["3d_512cubed"] = large_3d_leg(fw, 4, 1)
+    if "vendor" in legs:
+        extra["vendor_rocfft"] = vendor_leg(args.steps)
 
     cpu = None
     if not args.no_cpu:

@@ -32,3 +32,20 @@ def test_harness(built, name, args):
     r = subprocess.run([str(built / name)] + args, capture_output=True, text=True, timeout=120)
     print(name, " ".join(args), "->", r.stdout.strip())
     assert r.returncode == 0, (r.stdout, r.stderr)
+
+
+VENDOR = os.path.join(LIBDIR, "vendor_compare")
+
+
+@pytest.mark.skipif(not os.path.exists(VENDOR), reason="vendor_compare not built (needs rocFFT)")
+def test_vendor_compare_agrees_with_rocfft():
+    # the same forward transform through libtopefft and rocFFT, device-resident
+    import json
+    r = subprocess.run([VENDOR, "3", "2d4096", "1d65536", "batch32"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, (r.stdout, r.stderr)
+    rows = [json.loads(line) for line in r.stdout.splitlines() if line.startswith("{")]
+    assert [w["workload"] for w in rows] == ["2d4096", "1d65536", "batch32"]
+    for w in rows:
+        print(w)
+        assert w["rel_l2"] < (1e-5 if w["precision"] == "complex64" else 1e-12), w
+        assert w["topefft_us"] > 0 and w["rocfft_us"] > 0

@@ -447,9 +447,11 @@ class Planner {
         if (n <= kMaxFused) return add_pass((int)n, g, g, *cur, *cur, 1, 0, 0, nullptr, nullptr, 0, 0, 0, 0, axis_id);
         if (n > kMaxAxis) return TOPE_ERROR_INVALID_SIZE;
         // four-step: i = n1 + N1*n2.  Step 1 transforms n2 (length N2, stride
-        // N1*is) per (line, n1), multiplying output k2 by W_n^{n1*k2}; step 2
-        // transforms n1 (length N1, stride is) per (line, k2) and writes
-        // X[k2 + N2*k1] out of place.
+        // N1*is) per (line, n1), multiplying output k2 by W_n^{n1*k2}, into
+        // the other buffer; step 2 transforms n1 (length N1, stride is) per
+        // (line, k2) and writes X[k2 + N2*k1] back into this one, so the
+        // result ends where the data started (no swap, no copy for callers'
+        // arrays).
         const int L = ilog2ll(n);
         const long long N1 = 1LL << ((L + 1) / 2), N2 = n / N1;
         const int lo_bits = (L + 1) / 2;
@@ -459,22 +461,19 @@ class Planner {
         err = power_table(n, 1LL << lo_bits, n >> lo_bits, &hi);
         if (err != hipSuccess) return map_err(err);
 
-        // step 1 (in place): lines are (old o, old c, n1)
+        // step 1 (into the other buffer, same layout): lines are (old o, old c, n1)
         Fold f1 = fold(g, N1, g.is, g.is);
         Geo s1i = f1.in, s1o = f1.out;
         s1i.is = s1o.is = N1 * g.is;
-        int rc = add_pass((int)N2, s1i, s1o, *cur, *cur, f1.reps, f1.rep_in, f1.rep_out, lo, hi,
+        int rc = add_pass((int)N2, s1i, s1o, *cur, *cur ^ 1, f1.reps, f1.rep_in, f1.rep_out, lo, hi,
                           f1.new_in_o ? 1 : 0, f1.new_in_c ? 1 : 0, n, lo_bits, axis_id);
         if (rc) return rc;
-        // step 2 (out of place): lines are (old o, old c, k2)
+        // step 2 (back, transposed): lines are (old o, old c, k2)
         Fold f2 = fold(g, N2, N1 * g.is, g.is);
         Geo s2i = f2.in, s2o = f2.out;
         s2i.is = g.is;
         s2o.is = N2 * g.is;
-        rc = add_pass((int)N1, s2i, s2o, *cur, *cur ^ 1, f2.reps, f2.rep_in, f2.rep_out);
-        if (rc) return rc;
-        *cur ^= 1;
-        return TOPE_SUCCESS;
+        return add_pass((int)N1, s2i, s2o, *cur ^ 1, *cur, f2.reps, f2.rep_in, f2.rep_out);
     }
 
     // Non-power-of-two axis of length n (lines g) by Bluestein's identity
