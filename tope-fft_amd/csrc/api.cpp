@@ -72,6 +72,7 @@ struct Pass {
     long long dft_n = 0;
     const void *dft_roots = nullptr;
     const void *real_w = nullptr;  // PASS_REAL: W_n^k, k = 0..m (FP64)
+    const void *real_stw = nullptr;  // PASS_REAL fused with the row transform: stage table of (m, 16)
     int fixed_dir = -1;            // >= 0: the pass always runs in this direction (Bluestein's inner FFTs)
     int chirp_op = 0;              // PASS_CHIRP: CHIRP_PRE / CHIRP_MUL / CHIRP_POST
     long long chirp_n = 0, chirp_M = 0;
@@ -774,7 +775,10 @@ int launch_pass(PlanImpl *P, size_t i, hipStream_t s, int dir)
         const void *src = P->buf[fwd ? ps.in_buf : ps.out_buf];
         void *dst = P->buf[fwd ? ps.out_buf : ps.in_buf];
         hipError_t err =
-            launch_real_pass(fwd ? 0 : 1, src, dst, ps.real_w, P->real_m, P->real_pitch, P->real_rows, P->real_scale, s);
+            ps.real_stw ? launch_real_rows(fwd ? 0 : 1, src, dst, ps.real_stw, ps.real_w, P->real_m, P->real_pitch,
+                                           P->real_rows, P->real_scale, s)
+                        : launch_real_pass(fwd ? 0 : 1, src, dst, ps.real_w, P->real_m, P->real_pitch, P->real_rows,
+                                           P->real_scale, s);
         if (err != hipSuccess) return map_err(err);
     } else if (ps.kind == PASS_MFMA_TILE) {
         MfmaTileArgs m = ps.mt;
@@ -1071,8 +1075,11 @@ int build_real_plan(struct topeFFT *f, long long nx, long long ny, long long nz,
     if (err != hipSuccess) return fail(map_err(err));
     Planner pl(P, s);
     int cur = 0;
-    // X: m-point complex rows of the real array (buffer 0, in place)
-    int rc = pl.axis(m, Geo{1, 0, rows, m, 1}, &cur, 0);
+    // X: m-point complex rows of the real array and the half-spectrum step:
+    // one fused pass from buffer 0 into buffer 1 where a kernel covers m,
+    // else the row pass in place and the step on its own
+    const bool fused = real_rows_supported(m);
+    int rc = fused ? (int)TOPE_SUCCESS : pl.axis(m, Geo{1, 0, rows, m, 1}, &cur, 0);
     if (!rc && cur != 0) rc = TOPE_ERROR_UNSUPPORTED;
     if (!rc) {
         void *w = nullptr;
@@ -1084,6 +1091,16 @@ int build_real_plan(struct topeFFT *f, long long nx, long long ny, long long nz,
         ps.in_buf = 0;
         ps.out_buf = 1;
         ps.name = "real_half_spectrum";
+        if (fused) {
+            void *stw = nullptr;
+            err = pl.stage_table((int)m, 16, &stw);
+            if (err != hipSuccess) return fail(map_err(err));
+            ps.real_stw = stw;
+            ps.name = "real_rows_fused";
+            P->axis_tw[0].n = (int)m;
+            P->axis_tw[0].e = 16;
+            P->axis_tw[0].table = stw;
+        }
         P->passes.push_back(ps);
         cur = 1;
         // Y: columns kx < m of every z plane, then the Nyquist column kx = m

@@ -64,6 +64,130 @@ __global__ void c2r_pre_kernel(const cpx<double> *__restrict__ X, cpx<double> *_
     if (m - k != k && m - k < m) out(xm, xk, m - k);
 }
 
+// Fused rows: the m-point complex row transform and the half-spectrum step
+// in one launch (no intermediate array).  TB rows per block, P = N/E threads
+// per row, thread p owning elements p + P e as in the row axis passes.
+//   R2C (SIGN = -1): load z (the real row read as m complex values), run
+//   the radix stages, exchange Z through LDS so each thread also holds its
+//   mirrors Z[m - k], form X[k] = E[k] + W^k O[k] for its k and X[m] (thread
+//   0), store the row of m + 1 values at `pitch`.
+//   C2R (SIGN = +1): load X[k] and X[m - k] for its k, form
+//   Z[k] = s (E[k] + i O[k]), run the inverse radix stages, store z.
+template <int N, int E, int TB, int SIGN>
+__global__ void __launch_bounds__(TB *(N / E))
+real_rows_kernel(const cpx<double> *__restrict__ in, cpx<double> *__restrict__ out,
+                 const cpx<double> *__restrict__ stw, const cpx<double> *__restrict__ w, long long pitch,
+                 long long rows, double s)
+{
+    using SC = Sched<N, E>;
+    constexpr int P = SC::P;
+    constexpr long long m = N;
+    __shared__ __align__(16) unsigned char lds[Exchange<double, N, E, TB, false, true>::BYTES];
+    const int tid = threadIdx.x;
+    const int t = tid / P, p = tid % P;
+    const long long row = (long long)blockIdx.x * TB + t;
+    const bool valid = row < rows;
+    cpx<double> v[E];
+    if constexpr (SIGN < 0) {
+        const cpx<double> *z = in + row * m;
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = valid ? z[p + P * e] : cpx<double>{0, 0};
+    } else {
+        const cpx<double> *X = in + row * pitch;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const long long k = p + P * e;
+            cpx<double> xk{0, 0}, xm{0, 0};
+            if (valid) {
+                xk = X[k];
+                xm = X[m - k];
+            }
+            const cpx<double> b{xm.x, -xm.y};
+            const cpx<double> ev{0.5 * (xk.x + b.x), 0.5 * (xk.y + b.y)};
+            const cpx<double> d{0.5 * (xk.x - b.x), 0.5 * (xk.y - b.y)};
+            const cpx<double> o = cmulc(d, w[k]);
+            v[e] = cpx<double>{s * (ev.x - o.y), s * (ev.y + o.x)};
+        }
+    }
+    run_stages<double, N, E, TB, false, true, SIGN, 0>(v, p, t, stw, lds);
+    if constexpr (SIGN > 0) {
+        if (!valid) return;
+        cpx<double> *z = out + row * m;
+#pragma unroll
+        for (int e = 0; e < E; ++e) z[p + P * e] = v[e];
+    } else {
+        // mirrors Z[(m - k) mod m] through LDS, one component at a time
+        double *l = reinterpret_cast<double *>(lds);
+        const int base = lds_index<N, TB, false>(t, 0);
+        cpx<double> mir[E];
+#pragma unroll
+        for (int comp = 0; comp < 2; ++comp) {
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int k = p + P * e;
+                l[base + k + (k >> 4)] = comp ? v[e].y : v[e].x;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int k = (int)((m - (p + P * e)) & (m - 1));
+                const double val = l[base + k + (k >> 4)];
+                if (comp) mir[e].y = val;
+                else mir[e].x = val;
+            }
+        }
+        if (!valid) return;
+        cpx<double> *X = out + row * pitch;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const long long k = p + P * e;
+            const cpx<double> a = v[e], b{mir[e].x, -mir[e].y};
+            const cpx<double> ev{0.5 * (a.x + b.x), 0.5 * (a.y + b.y)};
+            const cpx<double> o{0.5 * (a.y - b.y), -0.5 * (a.x - b.x)};
+            const cpx<double> wo = cmul(w[k], o);
+            X[k] = cpx<double>{ev.x + wo.x, ev.y + wo.y};
+        }
+        if (p == 0) X[m] = cpx<double>{v[0].x - v[0].y, 0.0};  // k = m: E[0] - O[0] = Re Z0 - Im Z0
+    }
+}
+
+template <int N, int E, int TB>
+static hipError_t launch_rows_t(int dir, const void *in, void *out, const void *stw, const void *w,
+                                long long pitch, long long rows, double s, hipStream_t st)
+{
+    const long long grid = (rows + TB - 1) / TB;
+    if (grid > 0x7fffffffLL) return hipErrorInvalidValue;
+    const cpx<double> *i = static_cast<const cpx<double> *>(in);
+    cpx<double> *o = static_cast<cpx<double> *>(out);
+    const cpx<double> *t = static_cast<const cpx<double> *>(stw);
+    const cpx<double> *ww = static_cast<const cpx<double> *>(w);
+    if (dir == 0)
+        hipLaunchKernelGGL((real_rows_kernel<N, E, TB, -1>), dim3((unsigned)grid), dim3(TB * (N / E)), 0, st, i, o, t,
+                           ww, pitch, rows, s);
+    else
+        hipLaunchKernelGGL((real_rows_kernel<N, E, TB, 1>), dim3((unsigned)grid), dim3(TB * (N / E)), 0, st, i, o, t,
+                           ww, pitch, rows, s);
+    return hipGetLastError();
+}
+
+bool real_rows_supported(long long m) { return m >= 64 && m <= 4096 && (m & (m - 1)) == 0; }
+
+hipError_t launch_real_rows(int dir, const void *in, void *out, const void *stw, const void *w, long long m,
+                            long long pitch, long long rows, double s, hipStream_t st)
+{
+    switch (m) {
+    case 64: return launch_rows_t<64, 16, 16>(dir, in, out, stw, w, pitch, rows, s, st);
+    case 128: return launch_rows_t<128, 16, 16>(dir, in, out, stw, w, pitch, rows, s, st);
+    case 256: return launch_rows_t<256, 16, 16>(dir, in, out, stw, w, pitch, rows, s, st);
+    case 512: return launch_rows_t<512, 16, 8>(dir, in, out, stw, w, pitch, rows, s, st);
+    case 1024: return launch_rows_t<1024, 16, 4>(dir, in, out, stw, w, pitch, rows, s, st);
+    case 2048: return launch_rows_t<2048, 16, 2>(dir, in, out, stw, w, pitch, rows, s, st);
+    case 4096: return launch_rows_t<4096, 16, 1>(dir, in, out, stw, w, pitch, rows, s, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_real_pass(int dir, const void *in, void *out, const void *w, long long m, long long pitch,
                             long long rows, double s, hipStream_t st)
 {
