@@ -267,8 +267,8 @@ def cpu_baseline(x, dims, budget_s=12.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true",
                     help="uneven batch: skip the hipGraph-replay methods (for kernel-trace profiling)")
@@ -298,6 +298,9 @@ def main():
     dom_bytes = 2 * 16 * n2  # one read + one write of the array per pass
     achieved = gbs(dom_bytes, pass2[dom] * 1e-3)
 
+    # the side legs (reported beside the headline, not part of `value`) time
+    # at most 20 steps each so the default run stays short
+    side_steps = max(5, min(20, args.steps // 2))
     legs = set(args.legs.split(",")) if args.legs != "all" else {"1d", "3d256", "mp", "many", "3d512", "r2c", "vendor"}
     extra = {}
     if "1d" in legs:
@@ -308,7 +311,7 @@ def main():
         n3 = 256 ** 3
         x3, _ = workloads.planted_input(workloads.SEED_3D, dims3)
         p3 = topefft.Plan(fw, dims3, x3)
-        t3, pass3 = measure(fw, p3, max(5, args.steps // 2), args.warmup)
+        t3, pass3 = measure(fw, p3, side_steps, args.warmup)
         alg3 = 2 * 16 * n3 * 3
         p3.destroy()
         del x3
@@ -317,15 +320,15 @@ def main():
                                 "pass_ms": [round(float(m), 4) for m in pass3]}
 
     if "mp" in legs:
-        extra["mixed_precision_batch"] = mixed_precision_leg(fw, max(5, args.steps // 2), args.warmup)
+        extra["mixed_precision_batch"] = mixed_precision_leg(fw, side_steps, args.warmup)
     if "many" in legs:
-        extra["uneven_batch"] = uneven_batch_leg(fw, max(5, args.steps // 2), args.warmup, not args.no_graph)
+        extra["uneven_batch"] = uneven_batch_leg(fw, min(side_steps, 10), min(args.warmup, 3), not args.no_graph)
     if "r2c" in legs:
-        extra["r2c_2d_4096"] = real_2d_leg(fw, args.steps, args.warmup)
+        extra["r2c_2d_4096"] = real_2d_leg(fw, side_steps, args.warmup)
     if "3d512" in legs:
         extra["3d_512cubed"] = large_3d_leg(fw, 4, 1)
     if "vendor" in legs:
-        extra["vendor_rocfft"] = vendor_leg(args.steps)
+        extra["vendor_rocfft"] = vendor_leg(side_steps)
 
     cpu = None
     if not args.no_cpu:
