@@ -248,6 +248,25 @@ def vendor_leg(steps):
     return [json.loads(line) for line in r.stdout.splitlines() if line.startswith("{")]
 
 
+def cpu_library_fft(x, dims, cores):
+    """A tuned CPU FFT beside the restated reference: scipy.fft (pocketfft)
+    fft2 on the same input with `cores` worker threads, median of 5."""
+    try:
+        import scipy.fft
+    except ImportError:
+        return {"skipped": "scipy absent"}
+    a = np.asarray(x, dtype=np.complex128).reshape(dims[1], dims[0])
+    scipy.fft.fft2(a, workers=cores)  # warm-up
+    times = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        scipy.fft.fft2(a, workers=cores)
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    return {"library": f"scipy {scipy.__version__} fft2 (pocketfft)", "workers": cores,
+            "ms_per_transform": round(t * 1e3, 2), "gbs": round(gbs(2 * 16 * dims[0] * dims[1] * 2, t), 2)}
+
+
 def cpu_baseline(x, dims, budget_s=12.0):
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle_py
@@ -338,6 +357,7 @@ def main():
                          f"CPU restatement of the reference algorithm (radix-8 DIT, bit-reversal swap, "
                          f"one pass per stage), same input, {cores} OpenMP threads, "
                          f"{ct * 1e3:.1f} ms per transform"}
+        extra["cpu_library_fft"] = cpu_library_fft(x2, dims2, cores)
 
     flops2 = 5.0 * n2 * np.log2(n2)
     line = {
