@@ -118,6 +118,7 @@ void fw_release(struct topeFFT *f, void *owner);
 const AxisVariant *row_variant(int prec, int n);
 
 // Persistent uneven batch (many_kernel.hpp); args is ManyArgs<double|float>.
-hipError_t launch_many(int prec, const void *in, void *out, const void *args, int grid, int sign, hipStream_t s);
+hipError_t launch_many(int prec, int threads, const void *in, void *out, const void *args, int grid, int sign,
+                       hipStream_t s);
 
 }  // namespace tope

@@ -41,7 +41,8 @@ struct ManyImpl {
     int ntasks = 0;
     int seg0[16] = {}, seg1[16] = {};
     int sched[16] = {};
-    int grid = 0;
+    int grid = 0;        // blocks of the 1024-thread kernel (classes of P > 256)
+    int grid_small = 0;  // blocks of the 256-thread kernel (P <= 256)
     int method = TOPE_MANY_PERSISTENT;
     // per-transform baseline: single-transform tasks over an identity order
     ManyTask *d_single = nullptr;
@@ -50,6 +51,11 @@ struct ManyImpl {
     hipGraphExec_t gexec[4][2] = {};
     hipEvent_t e0 = nullptr, e1 = nullptr;  // timing only (autotuner): no system fence
     hipEvent_t x0 = nullptr, x1 = nullptr;  // around exec-with-download: fenced
+    // the 256-thread kernel runs beside the 1024-thread one on a second
+    // stream (fork/join through two events), filling the CUs the long
+    // classes leave idle at their tail
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
 };
 
 int map_err(hipError_t e) { return e == hipSuccess ? TOPE_SUCCESS : (int)e; }
@@ -70,6 +76,13 @@ int many_P(int L, bool e8)
     return n < 16 ? 1 : n / 16;
 }
 
+// Block size of the kernel that runs class L: 256 threads if one transform
+// of its schedule fits them, else 1024.
+int class_threads(const ManyImpl *M, int L)
+{
+    return many_P(L, M->sched[L] != 0) <= MANY_SMALL_THREADS ? MANY_SMALL_THREADS : MANY_THREADS;
+}
+
 void drop_graphs(ManyImpl *M)
 {
     for (auto &g : M->gexec)
@@ -88,7 +101,7 @@ hipError_t build_tasks(ManyImpl *M)
     std::vector<ManyTask> tasks;
     for (int L = 0; L < 16; ++L) M->seg0[L] = M->seg1[L] = 0;
     for (int L = MANY_LMAX; L >= MANY_LMIN; --L) {
-        const int tb = MANY_THREADS / many_P(L, M->sched[L] != 0);
+        const int tb = class_threads(M, L) / many_P(L, M->sched[L] != 0);
         const int first = (int)order.size();
         for (int i = 0; i < M->count; ++i)
             if (M->sizes[i] == (1 << L)) order.push_back(i);
@@ -121,14 +134,16 @@ hipError_t build_tasks(ManyImpl *M)
     return err;
 }
 
+// Arguments of one of the kernels (by block size): the segment bounds of
+// the classes it runs, empty segments for the others.
 template <typename R>
-ManyArgs<R> many_args(const ManyImpl *M, int only_class = -1)
+ManyArgs<R> many_args(const ManyImpl *M, int threads, int only_class = -1)
 {
     ManyArgs<R> a{};
     a.tasks = M->d_tasks;
     a.ntasks = M->ntasks;
     for (int L = 0; L < 16; ++L) {
-        const bool on = only_class < 0 || only_class == L;
+        const bool on = (only_class < 0 || only_class == L) && class_threads(M, L) == threads;
         a.seg0[L] = on ? M->seg0[L] : 0;
         a.seg1[L] = on ? M->seg1[L] : 0;
         a.sched[L] = M->sched[L];
@@ -140,17 +155,44 @@ ManyArgs<R> many_args(const ManyImpl *M, int only_class = -1)
     return a;
 }
 
+// The classes of each block size on their kernel (launched only if it has
+// tasks).  The two kernels touch disjoint transforms: the 1024-thread one
+// runs on the plan's stream, the 256-thread one on the side stream beside
+// it (fork/join), filling the CUs the long classes leave idle.
+// `grid` overrides the 1024-thread kernel's block count (autotuner).
 int launch_persistent(ManyImpl *M, int dir, hipStream_t s, int only_class = -1, int grid = 0)
 {
     const int sign = dir == FORWARD ? -1 : 1;
-    const int g = grid > 0 ? grid : M->grid;
-    hipError_t err;
-    if (M->prec == PREC_F64) {
-        ManyArgs<double> a = many_args<double>(M, only_class);
-        err = launch_many(PREC_F64, M->data, M->data, &a, g, sign, s);
-    } else {
-        ManyArgs<float> a = many_args<float>(M, only_class);
-        err = launch_many(PREC_F32, M->data, M->data, &a, g, sign, s);
+    const int sizes[2] = {MANY_THREADS, MANY_SMALL_THREADS};
+    bool any[2] = {false, false};
+    for (int L = MANY_LMIN; L <= MANY_LMAX; ++L)
+        if ((only_class < 0 || only_class == L) && M->seg1[L] > M->seg0[L])
+            for (int k = 0; k < 2; ++k)
+                if (class_threads(M, L) == sizes[k]) any[k] = true;
+    const int first = any[0] ? 0 : 1;
+    const bool fork = any[0] && any[1] && M->side;
+    hipError_t err = hipSuccess;
+    if (fork) {
+        err = hipEventRecord(M->fork, s);
+        if (err == hipSuccess) err = hipStreamWaitEvent(M->side, M->fork, 0);
+        if (err != hipSuccess) return map_err(err);
+    }
+    for (int k = 0; k < 2; ++k) {
+        if (!any[k]) continue;
+        const int g = k == 0 ? (grid > 0 ? grid : M->grid) : M->grid_small;
+        hipStream_t st = (fork && k != first) ? M->side : s;
+        if (M->prec == PREC_F64) {
+            ManyArgs<double> a = many_args<double>(M, sizes[k], only_class);
+            err = launch_many(PREC_F64, sizes[k], M->data, M->data, &a, g, sign, st);
+        } else {
+            ManyArgs<float> a = many_args<float>(M, sizes[k], only_class);
+            err = launch_many(PREC_F32, sizes[k], M->data, M->data, &a, g, sign, st);
+        }
+        if (err != hipSuccess) return map_err(err);
+    }
+    if (fork) {
+        err = hipEventRecord(M->join, M->side);
+        if (err == hipSuccess) err = hipStreamWaitEvent(s, M->join, 0);
     }
     return map_err(err);
 }
@@ -162,23 +204,24 @@ int launch_per_transform(ManyImpl *M, int dir, hipStream_t s)
     const int sign = dir == FORWARD ? -1 : 1;
     for (int i = 0; i < M->count; ++i) {
         const int L = ilog2i(M->sizes[i]);
+        const int nt = class_threads(M, L);
         hipError_t err;
         if (M->prec == PREC_F64) {
-            ManyArgs<double> a = many_args<double>(M, L);
+            ManyArgs<double> a = many_args<double>(M, nt, L);
             a.tasks = M->d_single + i;
             a.ntasks = 1;
             a.order = M->d_ident;
             a.seg0[L] = 0;
             a.seg1[L] = 1;
-            err = launch_many(PREC_F64, M->data, M->data, &a, 1, sign, s);
+            err = launch_many(PREC_F64, nt, M->data, M->data, &a, 1, sign, s);
         } else {
-            ManyArgs<float> a = many_args<float>(M, L);
+            ManyArgs<float> a = many_args<float>(M, nt, L);
             a.tasks = M->d_single + i;
             a.ntasks = 1;
             a.order = M->d_ident;
             a.seg0[L] = 0;
             a.seg1[L] = 1;
-            err = launch_many(PREC_F32, M->data, M->data, &a, 1, sign, s);
+            err = launch_many(PREC_F32, nt, M->data, M->data, &a, 1, sign, s);
         }
         if (err != hipSuccess) return map_err(err);
     }
@@ -219,6 +262,7 @@ void free_many(ManyImpl *M)
 {
     if (!M) return;
     if (M->stream) (void)hipStreamSynchronize(M->stream);
+    if (M->side) (void)hipStreamSynchronize(M->side);
     drop_graphs(M);
     for (void *p : M->allocs) (void)hipFree(p);
     if (M->data) (void)hipFree(M->data);
@@ -231,6 +275,9 @@ void free_many(ManyImpl *M)
     if (M->e1) (void)hipEventDestroy(M->e1);
     if (M->x0) (void)hipEventDestroy(M->x0);
     if (M->x1) (void)hipEventDestroy(M->x1);
+    if (M->fork) (void)hipEventDestroy(M->fork);
+    if (M->join) (void)hipEventDestroy(M->join);
+    if (M->side) (void)hipStreamDestroy(M->side);
     delete M;
 }
 
@@ -366,7 +413,8 @@ void topeManyPlanInit(struct topeFFT *f, struct topePlanMany *t, int count, cons
     }
     M->elems = acc;
     const int cus = device_cus();
-    M->grid = cus;  // one 1024-thread block per CU (LDS-bound for FP64)
+    M->grid = cus;            // one 1024-thread block per CU (LDS-bound for FP64)
+    M->grid_small = 4 * cus;  // four 256-thread blocks per CU (registers / LDS)
     // defaults: radix-16 schedules everywhere
     for (int L = 0; L < 16; ++L) M->sched[L] = 0;
     load_wisdom(M, cus);
@@ -378,7 +426,11 @@ void topeManyPlanInit(struct topeFFT *f, struct topePlanMany *t, int count, cons
     if ((long long)M->elems * (long long)M->elem >= 0x7fffffffLL) return fail(TOPE_ERROR_INVALID_SIZE);
 
     hipEvent_t p0 = nullptr, p1 = nullptr;
-    hipError_t err = hipEventCreate(&p0);
+    hipError_t err = hipStreamCreateWithFlags(&M->side, hipStreamNonBlocking);
+    if (err == hipSuccess) err = hipEventCreateWithFlags(&M->fork, hipEventDisableTiming);
+    if (err == hipSuccess) err = hipEventCreateWithFlags(&M->join, hipEventDisableTiming);
+    if (err != hipSuccess) return fail(map_err(err));
+    err = hipEventCreate(&p0);
     if (err == hipSuccess) err = hipEventCreate(&p1);
     if (err == hipSuccess) err = hipEventRecord(p0, s);
     if (err == hipSuccess) err = hipMalloc(&M->data, (size_t)M->elems * M->elem);

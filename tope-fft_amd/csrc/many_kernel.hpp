@@ -15,7 +15,13 @@
 
 namespace tope {
 
+// Two block sizes, by the threads a class's schedule needs per transform
+// (P = n/E): 1024 for P > 256 (a 16384-point FP64 transform fills one
+// block's registers, one block per CU), 256 for P <= 256 (four blocks per
+// CU, so one block's loads overlap another's arithmetic).  A 512-thread
+// tier for P = 512 measured slower on the 20,000-transform batch.
 constexpr int MANY_THREADS = 1024;
+constexpr int MANY_SMALL_THREADS = 256;
 constexpr int MANY_LMIN = 4, MANY_LMAX = 14;
 
 struct ManyTask {
@@ -37,12 +43,12 @@ struct ManyArgs {
     long long elems;        // packed buffer length (elements)
 };
 
-template <typename R, int N, int E, int SIGN>
+template <typename R, int N, int E, int SIGN, int NT>
 __device__ __forceinline__ void many_rows(const cpx<R> *in, cpx<R> *out, const ManyArgs<R> &a, const ManyTask &tk,
                                           void *lds)
 {
     constexpr int P = N / E;
-    constexpr int TB = MANY_THREADS / P;
+    constexpr int TB = NT / P;
     using SC = Sched<N, E>;
     constexpr int CB = (int)sizeof(cpx<R>);
     // Opaque to the optimiser: nothing lane-dependent (LDS bases, twiddle
@@ -69,31 +75,31 @@ __device__ __forceinline__ void many_rows(const cpx<R> *in, cpx<R> *out, const M
     }
 }
 
-// LDS: TB * (N + N/16) slots of one component for every class = 17408 words.
-template <typename R>
-constexpr int many_lds_bytes() { return MANY_THREADS / 16 * 17 * 16 * (int)sizeof(R); }
+// LDS: TB * (N + N/16) slots of one component for every class = NT * 17 words.
+template <typename R, int NT>
+constexpr int many_lds_bytes() { return NT / 16 * 17 * 16 * (int)sizeof(R); }
 
 // Dispatch one task of class L to its schedule (E = 8 only where n/8 threads
 // fit the block).
-template <typename R, int L, int SIGN>
+template <typename R, int L, int SIGN, int NT>
 __device__ __forceinline__ void many_class(const cpx<R> *in, cpx<R> *out, const ManyArgs<R> &a, const ManyTask &tk,
                                            void *lds, bool e8)
 {
     constexpr int N = 1 << L;
-    if constexpr (N >= 8 && N / 8 <= MANY_THREADS) {
+    if constexpr (N >= 8 && N / 8 <= NT) {
         if (e8) {
-            many_rows<R, N, 8, SIGN>(in, out, a, tk, lds);
+            many_rows<R, N, 8, SIGN, NT>(in, out, a, tk, lds);
             return;
         }
     }
-    many_rows<R, N, (N < 16 ? N : 16), SIGN>(in, out, a, tk, lds);
+    if constexpr ((N < 16 ? 1 : N / 16) <= NT) many_rows<R, N, (N < 16 ? N : 16), SIGN, NT>(in, out, a, tk, lds);
 }
 
 // Tasks of one class: this block takes global task indices b, b + G, ...
 // that fall in [seg0[L], seg1[L]) -- the same round-robin deal as one loop
 // over the whole list, but each loop body holds one class's code, which keeps
 // the register allocation of the 1024-thread block free of spills.
-template <typename R, int L, int SIGN>
+template <typename R, int L, int SIGN, int NT>
 __device__ __forceinline__ void many_segment(const cpx<R> *in, cpx<R> *out, const ManyArgs<R> &a, void *lds)
 {
     const int s0 = a.seg0[L], s1 = a.seg1[L];
@@ -107,26 +113,36 @@ __device__ __forceinline__ void many_segment(const cpx<R> *in, cpx<R> *out, cons
         tk.cls = L;
         tk.first = __builtin_amdgcn_readfirstlane(tk.first);
         tk.count = __builtin_amdgcn_readfirstlane(tk.count);
-        many_class<R, L, SIGN>(in, out, a, tk, lds, e8);
+        many_class<R, L, SIGN, NT>(in, out, a, tk, lds, e8);
         __syncthreads();  // the next task reuses the LDS
     }
 }
 
-template <typename R, int SIGN>
-__global__ void __launch_bounds__(MANY_THREADS) many_kernel(const cpx<R> *in, cpx<R> *out, const ManyArgs<R> a)
+// The classes a kernel holds code for (the host routes each class to one
+// kernel through the segment bounds it passes): 16384, 8192 and 4096
+// points (the last for its radix-8 schedule, 512 threads per transform) at
+// 1024 threads; every class up to 4096 points at 256, capped at 128 VGPRs
+// so four blocks share a CU.
+template <typename R, int SIGN, int NT>
+__global__ void __launch_bounds__(NT, NT >= 1024 ? 1 : 4) many_kernel(const cpx<R> *in, cpx<R> *out,
+                                                                     const ManyArgs<R> a)
 {
-    __shared__ __align__(16) unsigned char lds[many_lds_bytes<R>()];
-    many_segment<R, 14, SIGN>(in, out, a, lds);
-    many_segment<R, 13, SIGN>(in, out, a, lds);
-    many_segment<R, 12, SIGN>(in, out, a, lds);
-    many_segment<R, 11, SIGN>(in, out, a, lds);
-    many_segment<R, 10, SIGN>(in, out, a, lds);
-    many_segment<R, 9, SIGN>(in, out, a, lds);
-    many_segment<R, 8, SIGN>(in, out, a, lds);
-    many_segment<R, 7, SIGN>(in, out, a, lds);
-    many_segment<R, 6, SIGN>(in, out, a, lds);
-    many_segment<R, 5, SIGN>(in, out, a, lds);
-    many_segment<R, 4, SIGN>(in, out, a, lds);
+    __shared__ __align__(16) unsigned char lds[many_lds_bytes<R, NT>()];
+    if constexpr (NT >= 1024) {
+        many_segment<R, 14, SIGN, NT>(in, out, a, lds);
+        many_segment<R, 13, SIGN, NT>(in, out, a, lds);
+        many_segment<R, 12, SIGN, NT>(in, out, a, lds);
+    } else {
+        many_segment<R, 12, SIGN, NT>(in, out, a, lds);
+        many_segment<R, 11, SIGN, NT>(in, out, a, lds);
+        many_segment<R, 10, SIGN, NT>(in, out, a, lds);
+        many_segment<R, 9, SIGN, NT>(in, out, a, lds);
+        many_segment<R, 8, SIGN, NT>(in, out, a, lds);
+        many_segment<R, 7, SIGN, NT>(in, out, a, lds);
+        many_segment<R, 6, SIGN, NT>(in, out, a, lds);
+        many_segment<R, 5, SIGN, NT>(in, out, a, lds);
+        many_segment<R, 4, SIGN, NT>(in, out, a, lds);
+    }
 }
 
 }  // namespace tope
