@@ -295,8 +295,10 @@ void topeRealDestroy(struct topeFFT *f, struct topePlanReal *t);
  * `count` transforms with lengths sizes[i] (powers of two, 16 .. 16384)
  * stored back to back: transform i starts at element sum_{k<i} sizes[k]
  * (topeManyOffsets).  Methods:
- *   TOPE_MANY_PERSISTENT            one persistent kernel launch for all
- *   TOPE_MANY_PERSISTENT_GRAPH      that launch replayed from a hipGraph
+ *   TOPE_MANY_PERSISTENT            persistent kernels for all (one for the
+ *                                   lengths above 4096, one beside it for
+ *                                   the others)
+ *   TOPE_MANY_PERSISTENT_GRAPH      those launches replayed from a hipGraph
  *   TOPE_MANY_PER_TRANSFORM         one kernel launch per transform
  *   TOPE_MANY_PER_TRANSFORM_GRAPH   those launches replayed from one hipGraph
  * topeManyTune times the candidate radix schedules of every length class

@@ -1,6 +1,7 @@
 // C-ABI of the packed uneven batch: many transforms of different power-of-two
-// lengths stored back to back (offset table), run by one persistent kernel
-// launch, optionally replayed from a captured hipGraph, with a per-length
+// lengths stored back to back (offset table), run by persistent kernels
+// (one for the long lengths, one beside it on a side stream for the rest),
+// optionally replayed from a captured hipGraph, with a per-length
 // radix schedule chosen by an autotuner whose results persist in a text
 // "wisdom" file.  No reference counterpart: the reference can express this
 // only as a loop of 1D plans, one topeFFTInit each (src/fft1d.c:303-407).
