@@ -94,12 +94,23 @@ __device__ __forceinline__ long long xcd_block(int remap)
     return (b & 7) * (G >> 3) + (b >> 3);
 }
 
-// LDS index of element i of block-local transform t.  One pad slot per 16
+// LDS pad shift: element i sits at slot i + (i >> K).  Column tiles (TB
+// transforms interleaved) use K = 4.  Rows use K = 5 once a transform spans
+// at least 32 threads (P = N/E >= 32) and K = 4 below: with 64 four-byte
+// banks and eight-byte accesses served 32 lanes a cycle, a bank model of
+// every exchange puts the conflict cost of K = 4 at 1.50-2.00x the
+// conflict-free cycles for those rows and of K = 5 at 1.17-1.75x, while
+// K = 4 is the better one (1.00-1.50x) for the shorter rows.
+template <int N, int E, bool CFAST>
+constexpr int lds_pad_shift() { return (!CFAST && N / E >= 32) ? 5 : 4; }
+
+// LDS index of element i of block-local transform t: one pad slot per 2^K
 // elements breaks the power-of-two strides of the Stockham scatter.
-template <int N, int TB, bool CFAST>
+template <int N, int E, int TB, bool CFAST>
 __device__ __forceinline__ int lds_index(int t, int i)
 {
-    const int pi = i + (i >> 4);
+    constexpr int K = lds_pad_shift<N, E, CFAST>();
+    const int pi = i + (i >> K);
     if constexpr (CFAST) return pi * TB + t;
     else return t * (N + N / 16) + pi;
 }
@@ -107,14 +118,19 @@ __device__ __forceinline__ int lds_index(int t, int i)
 template <int N, int TB>
 constexpr int lds_slots() { return TB * (N + N / 16); }
 
-// The pad map i -> i + (i >> 4) is linear over every access group used
+// The pad map i -> i + (i >> K) is linear over every access group used
 // here: element p + P*e (P a power of two) and the radix-r scatter
-// base + m*Ns (Ns, r powers of two) never carry across a 16-boundary
+// base + m*Ns (Ns, r powers of two) never carry across a 2^K boundary
 // differently from their offset, so padded(base + d) = padded(base) +
-// d + (d >> 4) with d = P*e or m*Ns.  This lets every LDS access be one
-// base register plus a compile-time offset.
-template <int TB, bool CFAST>
-constexpr int lds_off(int d) { return (d + (d >> 4)) * (CFAST ? TB : 1); }
+// d + (d >> K) with d = P*e or m*Ns (checked for every compiled shape by
+// a CPU model of the exchanges).  This lets every LDS access be one base
+// register plus a compile-time offset.
+template <int N, int E, int TB, bool CFAST>
+constexpr int lds_off(int d)
+{
+    constexpr int K = lds_pad_shift<N, E, CFAST>();
+    return (d + (d >> K)) * (CFAST ? TB : 1);
+}
 
 // LDS exchange of the E register elements of every thread: write element
 // e to slot dst(e), then read slots p + P*e back.  SPLIT exchanges the real
@@ -287,38 +303,38 @@ __device__ __forceinline__ void run_stages(cpx<R> (&v)[E], int p, int t, const c
             for (int e = 0; e < E; ++e) v[e] = w[e];
         } else {
             constexpr int Ns = SC::span(S);
-            const int rbase = lds_index<N, TB, CFAST>(t, p);
+            const int rbase = lds_index<N, E, TB, CFAST>(t, p);
             int wbase[Q];
 #pragma unroll
-            for (int q = 0; q < Q; ++q) wbase[q] = lds_index<N, TB, CFAST>(t, stage_dst<N, E, S>(p + q * P, 0));
+            for (int q = 0; q < Q; ++q) wbase[q] = lds_index<N, E, TB, CFAST>(t, stage_dst<N, E, S>(p + q * P, 0));
             if constexpr (!SPLIT) {
                 cpx<R> *lds = (cpx<R> *)lds_raw;
                 if constexpr (S > 0) __syncthreads();
 #pragma unroll
                 for (int q = 0; q < Q; ++q)
 #pragma unroll
-                    for (int m = 0; m < r; ++m) lds[wbase[q] + lds_off<TB, CFAST>(m * Ns)] = v[q + m * Q];
+                    for (int m = 0; m < r; ++m) lds[wbase[q] + lds_off<N, E, TB, CFAST>(m * Ns)] = v[q + m * Q];
                 __syncthreads();
 #pragma unroll
-                for (int e = 0; e < E; ++e) v[e] = lds[rbase + lds_off<TB, CFAST>(P * e)];
+                for (int e = 0; e < E; ++e) v[e] = lds[rbase + lds_off<N, E, TB, CFAST>(P * e)];
             } else {
                 R *lds = (R *)lds_raw;
                 if constexpr (S > 0) __syncthreads();
 #pragma unroll
                 for (int q = 0; q < Q; ++q)
 #pragma unroll
-                    for (int m = 0; m < r; ++m) lds[wbase[q] + lds_off<TB, CFAST>(m * Ns)] = v[q + m * Q].x;
+                    for (int m = 0; m < r; ++m) lds[wbase[q] + lds_off<N, E, TB, CFAST>(m * Ns)] = v[q + m * Q].x;
                 __syncthreads();
 #pragma unroll
-                for (int e = 0; e < E; ++e) v[e].x = lds[rbase + lds_off<TB, CFAST>(P * e)];
+                for (int e = 0; e < E; ++e) v[e].x = lds[rbase + lds_off<N, E, TB, CFAST>(P * e)];
                 __syncthreads();
 #pragma unroll
                 for (int q = 0; q < Q; ++q)
 #pragma unroll
-                    for (int m = 0; m < r; ++m) lds[wbase[q] + lds_off<TB, CFAST>(m * Ns)] = v[q + m * Q].y;
+                    for (int m = 0; m < r; ++m) lds[wbase[q] + lds_off<N, E, TB, CFAST>(m * Ns)] = v[q + m * Q].y;
                 __syncthreads();
 #pragma unroll
-                for (int e = 0; e < E; ++e) v[e].y = lds[rbase + lds_off<TB, CFAST>(P * e)];
+                for (int e = 0; e < E; ++e) v[e].y = lds[rbase + lds_off<N, E, TB, CFAST>(P * e)];
             }
         }
         run_stages<R, N, E, TB, CFAST, SPLIT, SIGN, S + 1, TWL>(v, p, t, tw, lds_raw, twl);
@@ -419,16 +435,16 @@ __device__ __forceinline__ void axis_block(const cpx<R> *in, cpx<R> *out, const 
         // exchanged through LDS one component at a time (no extra registers)
         R *l = (R *)lds;
         const int mate = t ^ 1;
-        const int rbase = lds_index<N, TB, CFAST>(t, p), mbase = lds_index<N, TB, CFAST>(mate, p);
+        const int rbase = lds_index<N, E, TB, CFAST>(t, p), mbase = lds_index<N, E, TB, CFAST>(mate, p);
 #pragma unroll
         for (int comp = 0; comp < 2; ++comp) {
             __syncthreads();
 #pragma unroll
-            for (int e = 0; e < E; ++e) l[rbase + lds_off<TB, CFAST>(P * e)] = comp ? v[e].y : v[e].x;
+            for (int e = 0; e < E; ++e) l[rbase + lds_off<N, E, TB, CFAST>(P * e)] = comp ? v[e].y : v[e].x;
             __syncthreads();
 #pragma unroll
             for (int e = 0; e < E; ++e) {
-                const R m = l[mbase + lds_off<TB, CFAST>(P * e)];
+                const R m = l[mbase + lds_off<N, E, TB, CFAST>(P * e)];
                 R &mine = comp ? v[e].y : v[e].x;
                 mine = (t & 1) ? (m - mine) : (mine + m);
             }

@@ -118,7 +118,7 @@ real_rows_kernel(const cpx<double> *__restrict__ in, cpx<double> *__restrict__ o
     } else {
         // mirrors Z[(m - k) mod m] through LDS, one component at a time
         double *l = reinterpret_cast<double *>(lds);
-        const int base = lds_index<N, TB, false>(t, 0);
+        const int base = lds_index<N, E, TB, false>(t, 0);
         cpx<double> mir[E];
 #pragma unroll
         for (int comp = 0; comp < 2; ++comp) {
