@@ -20,6 +20,48 @@
 namespace tope {
 
 
+// Integer division by a runtime divisor without the emulated divide: a
+// float reciprocal estimate, exact to +-1 for the operands here (< 2^24),
+// then one correction.
+struct Div32 {
+    int d;
+    float r;
+};
+__device__ __forceinline__ Div32 mkdiv(int d) { return Div32{d, 1.0f / (float)d}; }
+__device__ __forceinline__ int divmod(int a, const Div32 &d, int &rem)
+{
+    int q = (int)((float)a * d.r);
+    int r = a - q * d.d;
+    if (r >= d.d) {
+        ++q;
+        r -= d.d;
+    } else if (r < 0) {
+        --q;
+        r += d.d;
+    }
+    rem = r;
+    return q;
+}
+// The same for 64-bit line numbers (< 2^40) through a double reciprocal.
+struct Div64 {
+    long long d;
+    double r;
+};
+__device__ __forceinline__ long long divmod64(long long a, const Div64 &d, long long &rem)
+{
+    long long q = (long long)((double)a * d.r);
+    long long r = a - q * d.d;
+    if (r >= d.d) {
+        ++q;
+        r -= d.d;
+    } else if (r < 0) {
+        --q;
+        r += d.d;
+    }
+    rem = r;
+    return q;
+}
+
 template <int SIGN>
 __device__ __forceinline__ cpx<double> root(const cpx<double> *__restrict__ w, long long i)
 {
@@ -68,9 +110,11 @@ __device__ __forceinline__ void mixed_stage(const cpx<double> *src, cpx<double> 
 {
     const int nb = n / r;  // butterflies per line
     const long long tws = (long long)n / ((long long)Ns * r);
+    const Div32 dnb = mkdiv(nb), dns = mkdiv(Ns);
     for (int bi = threadIdx.x; bi < tb * nb; bi += NT) {
-        const int t = bi / nb, j = bi - t * nb;
-        const int kk = j % Ns;
+        int j, kk;
+        const int t = divmod(bi, dnb, j);
+        const int jn = divmod(j, dns, kk);  // j = jn Ns + kk
         cpx<double> x[r];
         const cpx<double> *sl = src + t * n + j;
 #pragma unroll
@@ -81,7 +125,7 @@ __device__ __forceinline__ void mixed_stage(const cpx<double> *src, cpx<double> 
         }
         if constexpr (r == 3 || r == 5 || r == 7) dft_odd<r, SIGN>(x, roots, n / r);
         else dft<r, SIGN>(x);
-        cpx<double> *dl = dst + t * n + (j / Ns) * Ns * r + kk;
+        cpx<double> *dl = dst + t * n + jn * Ns * r + kk;
 #pragma unroll
         for (int m = 0; m < r; ++m) dl[m * Ns] = x[m];
     }
@@ -102,16 +146,17 @@ __global__ void __launch_bounds__(NT) mixed_kernel(const cpx<double> *in, cpx<do
     const long long l0 = xcd_block(cfast ? 1 : 0) * tb;
 
     // element idx of the block -> (line t, index k), and its global offset
+    const Div32 dtb = mkdiv(tb), dn = mkdiv(n);
+    const Div64 dC{a.C, 1.0 / (double)a.C};
     auto split = [&](int idx, int &t, int &k) {
-        if (cfast) {
-            t = idx % tb;
-            k = idx / tb;
-        } else {
-            t = idx / n;
-            k = idx % n;
-        }
+        if (cfast) k = divmod(idx, dtb, t);
+        else t = divmod(idx, dn, k);
     };
-    auto gaddr = [&](long long l, int k) { return (l / a.C) * a.os + (l % a.C) * a.cs + (long long)k * a.is; };
+    auto gaddr = [&](long long l, int k) {
+        long long c;
+        const long long o = divmod64(l, dC, c);
+        return o * a.os + c * a.cs + (long long)k * a.is;
+    };
 
     for (int idx = threadIdx.x; idx < span_elems; idx += NT) {
         int t, k;
