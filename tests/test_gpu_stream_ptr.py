@@ -18,7 +18,7 @@ def torch():
     return pytest.importorskip("torch")
 
 
-@pytest.mark.parametrize("dims", [(64, 32), (1 << 15,), (16, 8, 4)], ids=lambda d: "x".join(map(str, d)))
+@pytest.mark.parametrize("dims", [(64, 32), (1 << 15,), (16, 8, 4), (8192, 4), (4, 8, 8192)], ids=lambda d: "x".join(map(str, d)))
 def test_exec_ptr_torch_stream(oracle, torch, dims):
     n = int(np.prod(dims))
     rng = np.random.default_rng(n)
