@@ -103,6 +103,10 @@ __device__ __forceinline__ void dft_odd(cpx<double> (&x)[r], const cpx<double> *
     for (int k = 0; k < r; ++k) x[k] = y[k];
 }
 
+// LDS slot of element i of the block: one pad slot per 16 elements breaks
+// the power-of-two strides of the stage scatter (r, Ns) across the banks.
+__device__ __forceinline__ int lslot(int i) { return i + (i >> 4); }
+
 // One stage of compile-time radix r over the block's lines, LDS to LDS.
 template <int r, int SIGN, int NT>
 __device__ __forceinline__ void mixed_stage(const cpx<double> *src, cpx<double> *dst, int n, int tb, int Ns,
@@ -116,18 +120,18 @@ __device__ __forceinline__ void mixed_stage(const cpx<double> *src, cpx<double> 
         const int t = divmod(bi, dnb, j);
         const int jn = divmod(j, dns, kk);  // j = jn Ns + kk
         cpx<double> x[r];
-        const cpx<double> *sl = src + t * n + j;
+        const int si = t * n + j;
 #pragma unroll
-        for (int m = 0; m < r; ++m) x[m] = sl[m * nb];
+        for (int m = 0; m < r; ++m) x[m] = src[lslot(si + m * nb)];
         if (Ns > 1) {
 #pragma unroll
             for (int m = 1; m < r; ++m) x[m] = cmul(x[m], root<SIGN>(roots, (long long)kk * m * tws));
         }
         if constexpr (r == 3 || r == 5 || r == 7) dft_odd<r, SIGN>(x, roots, n / r);
         else dft<r, SIGN>(x);
-        cpx<double> *dl = dst + t * n + jn * Ns * r + kk;
+        const int di = t * n + jn * Ns * r + kk;
 #pragma unroll
-        for (int m = 0; m < r; ++m) dl[m * Ns] = x[m];
+        for (int m = 0; m < r; ++m) dst[lslot(di + m * Ns)] = x[m];
     }
 }
 
@@ -137,7 +141,8 @@ __device__ __forceinline__ void mixed_stage(const cpx<double> *src, cpx<double> 
 template <int SIGN, int CAP, int NT>
 __global__ void __launch_bounds__(NT) mixed_kernel(const cpx<double> *in, cpx<double> *out, const MixedArgs a)
 {
-    __shared__ cpx<double> lds[2 * CAP];
+    constexpr int SLOTS = CAP + CAP / 16;
+    __shared__ cpx<double> lds[2 * SLOTS];
     const int n = a.n, tb = a.tb;
     const int span_elems = tb * n;
     const bool cfast = a.is != 1;  // strided axis: neighbouring lanes take neighbouring lines
@@ -162,11 +167,11 @@ __global__ void __launch_bounds__(NT) mixed_kernel(const cpx<double> *in, cpx<do
         int t, k;
         split(idx, t, k);
         const long long l = l0 + t;
-        lds[t * n + k] = l < a.lines ? in[gaddr(l, k)] : cpx<double>{0, 0};
+        lds[lslot(t * n + k)] = l < a.lines ? in[gaddr(l, k)] : cpx<double>{0, 0};
     }
     __syncthreads();
 
-    cpx<double> *src = lds, *dst = lds + CAP;
+    cpx<double> *src = lds, *dst = lds + SLOTS;
     int Ns = 1;
     for (int s = 0; s < a.nst; ++s) {
         const int r = a.radix[s];
@@ -191,7 +196,7 @@ __global__ void __launch_bounds__(NT) mixed_kernel(const cpx<double> *in, cpx<do
         split(idx, t, k);
         const long long l = l0 + t;
         if (l < a.lines) {
-            const cpx<double> v = src[t * n + k];
+            const cpx<double> v = src[lslot(t * n + k)];
             out[gaddr(l, k)] = cpx<double>{v.x * a.scale, v.y * a.scale};
         }
     }
