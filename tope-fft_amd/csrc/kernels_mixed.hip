@@ -135,9 +135,11 @@ __device__ __forceinline__ void mixed_stage(const cpx<double> *src, cpx<double> 
     }
 }
 
-// CAP: elements of one LDS buffer (two buffers: 2 * CAP * 16 bytes); NT
-// threads (the single-line 4096-element blocks own a CU: more waves to
-// cover the twiddle-load latency).
+// CAP: elements of one LDS buffer (two buffers of CAP + CAP/16 slots); NT
+// threads: 512 for CAP = 2048 (two blocks, 16 waves per CU: 4 elements per
+// thread keep the latency of the stage loop covered; 256 threads measured
+// 10-12 % slower at 100^3 and 125^3), 1024 for the single-line 4096-element
+// blocks that own a CU.
 template <int SIGN, int CAP, int NT>
 __global__ void __launch_bounds__(NT) mixed_kernel(const cpx<double> *in, cpx<double> *out, const MixedArgs a)
 {
@@ -249,8 +251,8 @@ hipError_t launch_mixed(const void *in, void *out, const MixedArgs &a, int sign,
     cpx<double> *o = static_cast<cpx<double> *>(out);
     const dim3 g((unsigned)grid);
     if ((long long)a.tb * a.n <= 2048) {
-        if (sign < 0) hipLaunchKernelGGL((mixed_kernel<-1, 2048, 256>), g, dim3(256), 0, st, i, o, a);
-        else hipLaunchKernelGGL((mixed_kernel<1, 2048, 256>), g, dim3(256), 0, st, i, o, a);
+        if (sign < 0) hipLaunchKernelGGL((mixed_kernel<-1, 2048, 512>), g, dim3(512), 0, st, i, o, a);
+        else hipLaunchKernelGGL((mixed_kernel<1, 2048, 512>), g, dim3(512), 0, st, i, o, a);
     } else {
         if (sign < 0) hipLaunchKernelGGL((mixed_kernel<-1, MIXED_MAX_N, 1024>), g, dim3(1024), 0, st, i, o, a);
         else hipLaunchKernelGGL((mixed_kernel<1, MIXED_MAX_N, 1024>), g, dim3(1024), 0, st, i, o, a);
