@@ -260,7 +260,8 @@ void tope3DExecEx(struct topeFFT *f, struct topePlan3D *t, void *host, int dir);
  * topePlanExecPtr: transforms the caller's device array `data` (the plan's
  * element type and layout, interleaved complex) in place, asynchronously on
  * the framework's stream, without host copies; plan_impl is the plan
- * struct's impl field.  Not for real-input plans. */
+ * struct's impl field.  Not for real-input plans (topeRealExecPtr) or
+ * packed uneven batches (topeManyExecPtr). */
 void topeFFTSetStream(struct topeFFT *f, void *stream);
 void topePlanExecPtr(struct topeFFT *f, void *plan_impl, void *data, int dir);
 
@@ -288,6 +289,13 @@ struct topePlanReal {
 void topeRealPlanInit(struct topeFFT *f, struct topePlanReal *t, int x, int y, int z, const double *real);
 void topeRealExec(struct topeFFT *f, struct topePlanReal *t, double *host, int dir);
 void topeRealExecDevice(struct topeFFT *f, struct topePlanReal *t, int dir);
+/* topeRealExecPtr: the same transform on caller-owned device arrays, in
+ * the plan's device layout: `real` holds x*y*z doubles (realSize bytes),
+ * `spectrum` dataSize bytes (rows of x/2+1 complex128 at a pitch of `pitch`
+ * elements).  Asynchronous on the framework's stream, no host copies.
+ * FORWARD writes `spectrum` and may overwrite `real`; INVERSE writes `real`
+ * and may overwrite `spectrum`. */
+void topeRealExecPtr(struct topeFFT *f, struct topePlanReal *t, void *real, void *spectrum, int dir);
 void topeRealDestroy(struct topeFFT *f, struct topePlanReal *t);
 
 /* ---- packed batch of uneven power-of-two lengths (no reference
@@ -330,6 +338,11 @@ void topeManyPlanInit(struct topeFFT *f, struct topePlanMany *t, int count, cons
 void topeManySetMethod(struct topeFFT *f, struct topePlanMany *t, int method);
 void topeManyExec(struct topeFFT *f, struct topePlanMany *t, void *host, int dir);
 void topeManyExecDevice(struct topeFFT *f, struct topePlanMany *t, int dir);
+/* topeManyExecPtr: the plan's transforms on a caller-owned device array of
+ * the same packed layout (dataSize bytes), in place, asynchronously on the
+ * plan's stream; the graph methods run their launches without the graph
+ * (the graphs hold the plan's own array). */
+void topeManyExecPtr(struct topeFFT *f, struct topePlanMany *t, void *data, int dir);
 long long topeManyOffsets(struct topePlanMany *t, long long *offsets);
 void topeManyGetSchedule(struct topePlanMany *t, int *radix_by_log2, int *grid);
 int topeManyTune(struct topeFFT *f, struct topePlanMany *t, const char *wisdom_path, int reps);

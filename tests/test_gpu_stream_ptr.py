@@ -1,12 +1,14 @@
 """Caller streams and caller-owned device arrays (topeFFTSetStream,
-topePlanExecPtr): torch CUDA tensors transformed in place on torch's
-current stream, against the FP64 oracle (rel-L2 <= 1e-12), for an in-place
-schedule (2D 64 x 32) and one whose last step is out of place (1D 2^15,
-four-step), and a complex64 plan."""
+topePlanExecPtr, topeRealExecPtr, topeManyExecPtr): torch CUDA tensors
+transformed in place on torch's current stream, against the FP64 oracle
+(rel-L2 <= 1e-12), for an in-place schedule (2D 64 x 32) and one whose last
+step is out of place (1D 2^15, four-step), a complex64 plan, real-input
+plans and packed uneven batches under every method."""
 import numpy as np
 import pytest
 
 import topefft
+import workloads
 from conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
@@ -65,5 +67,89 @@ def test_exec_ptr_errors():
     p = topefft.Plan(fw, (16, 16), np.zeros(256, complex))
     with pytest.raises(topefft.TopeError):
         p.exec_ptr(0, topefft.FORWARD)
+    p.destroy()
+    fw.destroy()
+
+
+@pytest.mark.parametrize("dims", [(256, 64), (64, 8, 4), (8192,), (16, 16)], ids=lambda d: "x".join(map(str, d)))
+def test_real_exec_ptr(oracle, torch, dims):
+    # topeRealExecPtr on torch arrays in the plan's device layout; the
+    # output arrays start as NaN, so every element the result needs must be
+    # written, and the plan's own arrays stay zero
+    n = int(np.prod(dims))
+    nx = dims[0]
+    rows = n // nx
+    x = np.random.default_rng(n + 7).uniform(-1.0, 1.0, n)
+    fw = topefft.Framework()
+    p = topefft.RealPlan(fw, dims, np.zeros(n))
+    pitch = p.t.pitch
+    assert p.t.dataSize == pitch * rows * 16 and p.t.realSize == n * 8
+    real = torch.from_numpy(x.copy()).to("cuda")
+    spec = torch.full((pitch * rows,), complex(np.nan, np.nan), dtype=torch.complex128, device="cuda")
+    torch.cuda.synchronize()
+    p.exec_ptr(real.data_ptr(), spec.data_ptr(), topefft.FORWARD)
+    fw.synchronize()
+    X = spec.cpu().numpy().reshape(rows, pitch)[:, : nx // 2 + 1].reshape(-1)
+    back_t = torch.full((n,), np.nan, dtype=torch.float64, device="cuda")
+    torch.cuda.synchronize()
+    p.exec_ptr(back_t.data_ptr(), spec.data_ptr(), topefft.INVERSE)
+    fw.synchronize()
+    back = back_t.cpu().numpy()
+    own = p.forward()
+    p.destroy()
+    fw.destroy()
+    full = oracle.dft(x.astype(np.complex128), dims, topefft.FORWARD).reshape(rows, nx)
+    assert rel_l2(X, full[:, : nx // 2 + 1].reshape(-1)) < 1e-12
+    assert rel_l2(back, x) < 1e-12
+    assert not np.any(own)
+
+
+def test_many_exec_ptr(oracle, torch):
+    # topeManyExecPtr with every method (the graph methods launch without
+    # their graph): identical results, the oracle on two transforms per
+    # length, and the plan's own array untouched
+    sizes = workloads.uneven_sizes(0x5EED, 60, 4, 14)
+    total = int(sizes.sum())
+    x = workloads.complex_uniform(0x5EEE, total)
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    outs = {}
+    for m in (topefft.MANY_PERSISTENT, topefft.MANY_PERSISTENT_GRAPH, topefft.MANY_PER_TRANSFORM,
+              topefft.MANY_PER_TRANSFORM_GRAPH):
+        fw = topefft.Framework()
+        q = topefft.ManyPlan(fw, sizes, np.zeros(total, complex))
+        q.set_method(m)
+        t = torch.from_numpy(x.copy()).to("cuda")
+        torch.cuda.synchronize()
+        q.exec_ptr(t.data_ptr(), topefft.FORWARD)
+        fw.synchronize()
+        fwd = t.cpu().numpy()
+        q.exec_ptr(t.data_ptr(), topefft.INVERSE)
+        fw.synchronize()
+        outs[m] = (fwd, t.cpu().numpy())
+        assert not np.any(q.exec(topefft.FORWARD)), topefft.MANY_METHOD_NAMES[m]
+        q.destroy()
+        fw.destroy()
+    fwd, back = outs[topefft.MANY_PERSISTENT]
+    seen = {}
+    for n, o in zip(sizes, offs):
+        if seen.get(n, 0) < 2:
+            seen[n] = seen.get(n, 0) + 1
+            assert rel_l2(fwd[o:o + n], oracle.dft(x[o:o + n], (int(n),), topefft.FORWARD)) < 1e-12, n
+    assert rel_l2(back, x) < 1e-12
+    for m, (f, b) in outs.items():
+        assert np.array_equal(f, fwd) and np.array_equal(b, back), topefft.MANY_METHOD_NAMES[m]
+
+
+def test_real_and_many_exec_ptr_errors():
+    fw = topefft.Framework()
+    p = topefft.RealPlan(fw, (16, 4), np.zeros(64))
+    with pytest.raises(topefft.TopeError):
+        p.exec_ptr(0, p.t.data, topefft.FORWARD)
+    with pytest.raises(topefft.TopeError):
+        p.exec_ptr(p.t.real, p.t.data, 7)
+    q = topefft.ManyPlan(fw, np.array([16, 32], np.int32), np.zeros(48, complex))
+    with pytest.raises(topefft.TopeError):
+        q.exec_ptr(0, topefft.FORWARD)
+    q.destroy()
     p.destroy()
     fw.destroy()

@@ -1776,6 +1776,30 @@ void topeRealExecDevice(struct topeFFT *f, struct topePlanReal *t, int dir)
     f->error = (P && P->real) ? exec_device(f, P, dir) : TOPE_ERROR_NOT_INITIALIZED;
 }
 
+void topeRealExecPtr(struct topeFFT *f, struct topePlanReal *t, void *real, void *spectrum, int dir)
+{
+    if (!f || !t) return;
+    PlanImpl *P = impl_of(t->impl);
+    FrameworkImpl *fw = fw_of(f);
+    if (!P || !P->real || !fw || P->fw != fw) {
+        f->error = TOPE_ERROR_NOT_INITIALIZED;
+        return;
+    }
+    if (!real || !spectrum || (dir != FORWARD && dir != INVERSE)) {
+        f->error = TOPE_ERROR_INVALID_ARG;
+        return;
+    }
+    // the plan's passes run on the caller's arrays in place of its own
+    // (buffer 0: real array, buffer 1: half spectrum at the plan's pitch)
+    void *own0 = P->buf[0], *own1 = P->buf[1];
+    P->buf[0] = real;
+    P->buf[1] = spectrum;
+    const int rc = enqueue(P, fw->stream, dir);
+    P->buf[0] = own0;
+    P->buf[1] = own1;
+    f->error = rc;
+}
+
 void topeRealDestroy(struct topeFFT *f, struct topePlanReal *t)
 {
     if (!f || !t) return;

@@ -524,6 +524,29 @@ void topeManyExecDevice(struct topeFFT *f, struct topePlanMany *t, int dir)
     f->error = launch_method(M, M->method, dir, M->stream);
 }
 
+void topeManyExecPtr(struct topeFFT *f, struct topePlanMany *t, void *data, int dir)
+{
+    ManyImpl *M = impl_of(t);
+    if (!f || !M || M->fw != fw_handle(f)) {
+        if (f) f->error = TOPE_ERROR_INVALID_ARG;
+        return;
+    }
+    if (!data || (dir != FORWARD && dir != INVERSE)) {
+        f->error = TOPE_ERROR_INVALID_ARG;
+        return;
+    }
+    // the graphs hold the plan's own array: the graph methods run their
+    // launches directly here
+    int method = M->method;
+    if (method == TOPE_MANY_PERSISTENT_GRAPH) method = TOPE_MANY_PERSISTENT;
+    if (method == TOPE_MANY_PER_TRANSFORM_GRAPH) method = TOPE_MANY_PER_TRANSFORM;
+    void *own = M->data;
+    M->data = data;
+    const int rc = launch_method(M, method, dir, M->stream);
+    M->data = own;
+    f->error = rc;
+}
+
 void topeManyExec(struct topeFFT *f, struct topePlanMany *t, void *host, int dir)
 {
     ManyImpl *M = impl_of(t);

@@ -94,6 +94,7 @@ SIGNATURES = {
     "topeRealPlanInit": (None, [_F, _PR, C.c_int, C.c_int, C.c_int, _DP]),
     "topeRealExec": (None, [_F, _PR, _DP, C.c_int]),
     "topeRealExecDevice": (None, [_F, _PR, C.c_int]),
+    "topeRealExecPtr": (None, [_F, _PR, C.c_void_p, C.c_void_p, C.c_int]),
     "topeRealDestroy": (None, [_F, _PR]),
     "topeFFTInit": (None, [_F]),
     "topeFFTDestroy": (None, [_F]),
@@ -133,6 +134,7 @@ SIGNATURES = {
     "topeManySetMethod": (None, [_F, _PM, C.c_int]),
     "topeManyExec": (None, [_F, _PM, C.c_void_p, C.c_int]),
     "topeManyExecDevice": (None, [_F, _PM, C.c_int]),
+    "topeManyExecPtr": (None, [_F, _PM, C.c_void_p, C.c_int]),
     "topeManyOffsets": (C.c_longlong, [_PM, C.POINTER(C.c_longlong)]),
     "topeManyGetSchedule": (None, [_PM, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "topeManyTune": (C.c_int, [_F, _PM, C.c_char_p, C.c_int]),
@@ -420,6 +422,14 @@ class RealPlan:
         self.lib.topeRealExecDevice(C.byref(self.fw.f), C.byref(self.t), direction)
         self.fw.check("topeRealExecDevice")
 
+    def exec_ptr(self, real_ptr, spectrum_ptr, direction=FORWARD):
+        """Transform caller-owned device arrays in the plan's device layout
+        (real: realSize bytes; spectrum: dataSize bytes, rows at `pitch`),
+        asynchronously on the framework's stream."""
+        self.lib.topeRealExecPtr(C.byref(self.fw.f), C.byref(self.t), C.c_void_p(real_ptr),
+                                 C.c_void_p(spectrum_ptr), direction)
+        self.fw.check("topeRealExecPtr")
+
     def pass_count(self):
         return self.lib.topePlanPassCount(self.t.impl)
 
@@ -471,6 +481,12 @@ class ManyPlan:
     def exec_device(self, direction=FORWARD):
         self.lib.topeManyExecDevice(C.byref(self.fw.f), C.byref(self.t), direction)
         self.fw.check("topeManyExecDevice")
+
+    def exec_ptr(self, device_ptr, direction=FORWARD):
+        """Transform a caller-owned device array of the packed layout in
+        place, asynchronously on the plan's stream."""
+        self.lib.topeManyExecPtr(C.byref(self.fw.f), C.byref(self.t), C.c_void_p(device_ptr), direction)
+        self.fw.check("topeManyExecPtr")
 
     def schedule(self):
         r = (C.c_int * 16)()
