@@ -1027,8 +1027,8 @@ void free_plan(PlanImpl *P)
 // complex), buf[1] the half spectrum: rows of x/2 + 1 elements at a pitch of
 // x/2 + 8 elements (128-byte aligned rows for the strided passes).  FORWARD
 // (R2C): m-point rows in buf[0], post-processing into buf[1], then Y and Z
-// in buf[1] -- the columns 0..m-1 in 4-column tiles, the Nyquist column m
-// on its own; INVERSE (C2R) runs the same passes in reverse order, the
+// in buf[1] over the columns 0..m (the Nyquist column m in a tile of its
+// own that runs first); INVERSE (C2R) runs the same passes in reverse order, the
 // pre-processing scaled by 1/(m y z) so the result carries 1/(x y z).
 int build_real_plan(struct topeFFT *f, long long nx, long long ny, long long nz, const double *real, PlanImpl **out,
                     tope_ulong *pre_ns, tope_ulong *mem_ns)
@@ -1103,22 +1103,23 @@ int build_real_plan(struct topeFFT *f, long long nx, long long ny, long long nz,
         }
         P->passes.push_back(ps);
         cur = 1;
-        // Y: columns kx < m of every z plane, then the Nyquist column kx = m
-        // (a single-column pass based at element m); Z likewise per y row
+        // Y: the m + 1 columns kx <= m of every z plane in one pass; its
+        // last tile holds only the Nyquist column kx = m (its other lanes
+        // are masked and touch no memory) and is rotated to run first, so it
+        // overlaps the full tiles instead of trailing them.  Z likewise per
+        // y row.
         auto axis2 = [&](long long n, Geo g, int id) {
             if (n == 1) return (int)TOPE_SUCCESS;
+            const size_t before = P->passes.size();
             int r = pl.axis(n, g, &cur, id);
             if (r) return r;
-            g.C = 1;
-            const size_t before = P->passes.size();
-            r = pl.axis(n, g, &cur, -1);
-            if (r) return r;
             if (P->passes.size() != before + 1) return (int)TOPE_ERROR_UNSUPPORTED;
-            P->passes.back().base = m;
+            Pass &ps = P->passes.back();
+            ps.a.rot = ps.a32.rot = ps.a.tiles - 1;
             return (int)TOPE_SUCCESS;
         };
-        rc = axis2(ny, Geo{nz, pitch * ny, m, 1, pitch}, 1);
-        if (!rc) rc = axis2(nz, Geo{ny, pitch, m, 1, pitch * ny}, 2);
+        rc = axis2(ny, Geo{nz, pitch * ny, m + 1, 1, pitch}, 1);
+        if (!rc) rc = axis2(nz, Geo{ny, pitch, m + 1, 1, pitch * ny}, 2);
         if (!rc && cur != 1) rc = TOPE_ERROR_UNSUPPORTED;
     }
     if (rc) return fail(rc);

@@ -73,6 +73,8 @@ struct AxisArgs {
     int pt_shift;
     R scale;    // multiplies the output (1/N on the last inverse pass, else 1)
     int xcd;    // 1: remap blocks so each XCD walks a contiguous range of tiles
+    int rot;    // 0 <= rot < tiles: logical tile t of an outer index runs tile
+                // (t + rot) mod tiles (a partial last tile can go first)
     long long in_elems, out_elems;  // buffer sizes (bounds of the buffer resources)
     // DIF2 row passes: block-local transform t = 2q + h is row (tile*TB/2 + q)
     // shifted by h*pair elements; afterwards the two rows of each pair are
@@ -85,13 +87,16 @@ struct AxisArgs {
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own
 // L2.  Strided tiles that are neighbours in memory share 128-byte lines, so
-// with remapping block b runs logical tile (b % 8) * (G / 8) + b / 8: the
-// blocks an XCD receives back to back are adjacent tiles.
+// with remapping the blocks an XCD receives back to back are adjacent
+// tiles: with G = 8q + r blocks, XCD x < r takes q + 1 blocks and the others
+// q, each a contiguous range of logical tiles (block b = x + 8k runs tile
+// k plus the blocks of the XCDs before x; a bijection on [0, G)).
 __device__ __forceinline__ long long xcd_block(int remap)
 {
     const long long b = blockIdx.x, G = gridDim.x;
-    if (!remap || (G & 7)) return b;
-    return (b & 7) * (G >> 3) + (b >> 3);
+    if (!remap) return b;
+    const long long q = G >> 3, r = G & 7, x = b & 7, k = b >> 3;
+    return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
 
 // LDS pad shift: element i sits at slot i + (i >> K).  Column tiles (TB
@@ -391,7 +396,8 @@ __device__ __forceinline__ void axis_block(const cpx<R> *in, cpx<R> *out, const 
         for (int i = tid; i < TwLds<N, E>::TOTAL; i += TB * P) twl[i] = a.tw[TwLds<N, E>::src(i)];
     }
     const long long o = blk / a.tiles;
-    const int tile = (int)(blk - o * a.tiles);
+    int tile = (int)(blk - o * a.tiles) + a.rot;
+    if (tile >= a.tiles) tile -= a.tiles;
     // DIF2: TB/2 row pairs per block, transform t = 2q + h
     constexpr int TROWS = DIF2 ? TB / 2 : TB;
     const int c0 = tile * TROWS;
