@@ -405,3 +405,24 @@ def test_3d_512_fp64_and_complex64():
     err = rel_l2(X32.astype(np.complex128), X)
     print(f"512^3 complex64 vs FP64 rel-L2 {err:.3e}")
     assert err < 1e-6
+
+
+def test_3d_256_row_tiles_complex64_against_fp64():
+    # 256^3 takes the 4-row tiles for its 256-point rows (shapes of at least
+    # 8192 such blocks), in FP64 and complex64; the complex64 plan agrees
+    # with the FP64 one and both round-trip
+    dims = (256, 256, 256)
+    x = workloads.complex_uniform(workloads.SEED_3D + 11, 256 ** 3)
+    fw = topefft.Framework()
+    p = topefft.Plan(fw, dims, x)
+    q = topefft.Plan(fw, dims, x.astype(np.complex64), precision=topefft.PREC_F32)
+    names64, names32 = p.pass_names(), q.pass_names()
+    X = p.exec(topefft.FORWARD)
+    back = p.exec(topefft.INVERSE)
+    X32 = q.exec(topefft.FORWARD).astype(np.complex128)
+    q.destroy()
+    p.destroy()
+    assert names64[0] == "axis_double_n256_e16_tb4_row", names64
+    assert names32[0] == "axis_float_n256_e16_tb4_row", names32
+    assert rel_l2(back, x) < TOL
+    assert rel_l2(X32, X) < 1e-6

@@ -182,19 +182,27 @@ const AxisVariant *find_variant_named(const char *name)
 // split tiles 114 us vs 140 us for 2-column tiles; 3D 256^3 columns 78-83 us
 // with 32-column split tiles vs 83-87 us (16, 64); 3D 512^3 FP64 Z pass
 // 887 us with 16-column split tiles vs 1007 us (8) and 925 us (32), complex64
-// Z 443 us with 32-column split tiles vs 701 us (8).
-const char *const kPreferred[] = {
-    "axis_double_n4096_e16_tb1_row_split",
-    "axis_double_n4096_e16_tb4_col_split",
-    "axis_double_n2048_e16_tb4_col_split",
-    "axis_double_n256_e16_tb32_col_split",
-    "axis_double_n512_e16_tb16_col_split",
-    "axis_float_n512_e16_tb32_col_split",
-    "axis_float_n256_e16_tb32_col_split",
-    "axis_float_n4096_e16_tb1_row_split",
+// Z 443 us with 32-column split tiles vs 701 us (8).  256-point rows in
+// 4-row tiles, only for shapes of at least 8192 such blocks: 256^3 202.4 ->
+// 197.3 us (complex64 104.7 -> 103.5 us), but 256 x 4096 (1024 blocks, the
+// array in cache) 28.3 -> 34.1 us.
+struct Preferred {
+    const char *name;
+    long long min_blocks;
 };
-
 constexpr long long kMinBlocks = 256;  // one block per CU at least, when the shape allows
+const Preferred kPreferred[] = {
+    {"axis_double_n4096_e16_tb1_row_split", kMinBlocks},
+    {"axis_double_n4096_e16_tb4_col_split", kMinBlocks},
+    {"axis_double_n2048_e16_tb4_col_split", kMinBlocks},
+    {"axis_double_n256_e16_tb32_col_split", kMinBlocks},
+    {"axis_double_n512_e16_tb16_col_split", kMinBlocks},
+    {"axis_double_n256_e16_tb4_row", 8192},
+    {"axis_float_n512_e16_tb32_col_split", kMinBlocks},
+    {"axis_float_n256_e16_tb32_col_split", kMinBlocks},
+    {"axis_float_n4096_e16_tb1_row_split", kMinBlocks},
+    {"axis_float_n256_e16_tb4_row", 8192},
+};
 
 // Variant choice: a measured preference if the line count fills its tile;
 // otherwise strided tiles take the widest tile that the line count fills
@@ -212,10 +220,10 @@ const AxisVariant *select_variant(int prec, int n, bool cfast, long long lines, 
     }
     // blocks a tile width gives for this shape (small shapes: keep the CUs busy)
     auto blocks = [&](const AxisVariant *v) { return outer * ((lines + v->tb - 1) / v->tb); };
-    for (const char *name : kPreferred) {
-        const AxisVariant *v = find_variant_named(name);
+    for (const Preferred &pr : kPreferred) {
+        const AxisVariant *v = find_variant_named(pr.name);
         if (v && v->prec == prec && v->n == n && v->cfast == cfast && v->dif2 == dif2 && v->tb <= lines &&
-            blocks(v) >= kMinBlocks)
+            blocks(v) >= pr.min_blocks)
             return v;
     }
     int cnt = 0;
