@@ -261,7 +261,10 @@ void tope3DExecEx(struct topeFFT *f, struct topePlan3D *t, void *host, int dir);
  * element type and layout, interleaved complex) in place, asynchronously on
  * the framework's stream, without host copies; plan_impl is the plan
  * struct's impl field.  Not for real-input plans (topeRealExecPtr) or
- * packed uneven batches (topeManyExecPtr). */
+ * packed uneven batches (topeManyExecPtr).  The ExecPtr calls enqueue the
+ * plan's passes on the caller's arrays and return; the plan can take the
+ * next call at once (its own arrays are left untouched).  One host thread at
+ * a time per framework and its plans. */
 void topeFFTSetStream(struct topeFFT *f, void *stream);
 void topePlanExecPtr(struct topeFFT *f, void *plan_impl, void *data, int dir);
 
