@@ -270,7 +270,7 @@ void topePlanExecPtr(struct topeFFT *f, void *plan_impl, void *data, int dir);
 
 /* ---- real-input transforms (no reference counterpart; the reference
  * declares R2C but runs C2C) ---------------------------------------------
- * x (even, x/2 <= 4096) by y by z (<= 4096 each; 1 for unused axes), powers
+ * x (even, x/2 <= 2^24) by y by z (<= 4096 each; 1 for unused axes), powers
  * of two, FP64, x fastest.  PlanInit uploads the real array (x*y*z doubles).
  * topeRealExec FORWARD (R2C) transforms the device real array into the half
  * spectrum, (x/2+1)*y*z complex128 in the layout of numpy's rfftn (kx

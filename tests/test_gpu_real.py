@@ -1,6 +1,8 @@
 """Real-input transforms (topeReal*): R2C half spectrum against the FP64
 oracle's full complex transform (sliced to kx <= nx/2), C2R by round trip and
-against the oracle's inverse, closed forms, and the size limits.  Bar:
+against the oracle's inverse, rows above 8192 through the four-step (full
+comparison up to 32768, sampled bins at 2^22), closed forms, and the size
+limits.  Bar:
 rel-L2 <= 1e-12, as for the complex FP64 path."""
 import numpy as np
 import pytest
@@ -10,7 +12,10 @@ from conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(2,), (4,), (16,), (256,), (4096,), (8192,), (8, 4), (64, 32), (256, 256), (2, 8),
+# x above 8192: the row transform runs as the four-step (two passes) and the
+# half-spectrum step on its own
+SHAPES = [(2,), (4,), (16,), (256,), (4096,), (8192,), (16384,), (32768,), (16384, 4), (16384, 2, 2),
+          (8, 4), (64, 32), (256, 256), (2, 8),
           (16, 8, 4), (32, 32, 32), (2, 4, 8), (128, 2, 16)]
 
 
@@ -90,6 +95,26 @@ def test_real_plan_errors():
         topefft.RealPlan(fw, (6,), np.zeros(6))
     assert e.value.code == topefft.TOPE_ERROR_INVALID_SIZE
     with pytest.raises(topefft.TopeError) as e:
-        topefft.RealPlan(fw, (16384,), np.zeros(16384))
+        topefft.RealPlan(fw, (16, 8192), np.zeros(16 * 8192))
     assert e.value.code == topefft.TOPE_ERROR_UNSUPPORTED
     fw.destroy()
+
+
+def test_r2c_large_four_step_bins(oracle):
+    # x = 2^22: sampled bins of the half spectrum against the oracle's direct
+    # DFT, and the C2R round trip
+    n = 1 << 22
+    x = real_input((n,), 77)
+    fw = topefft.Framework()
+    p = topefft.RealPlan(fw, (n,), x)
+    names = p.pass_names()
+    X = p.forward()
+    back = p.inverse()
+    p.destroy()
+    fw.destroy()
+    assert len(names) == 3 and names[-1] == "real_half_spectrum", names
+    rng = np.random.default_rng(9)
+    bins = np.concatenate([[0, 1, n // 2, n // 2 - 1, n // 4], rng.integers(0, n // 2 + 1, 24)])
+    want = oracle.dft_bins(x.astype(np.complex128), (n,), bins, topefft.FORWARD)
+    assert rel_l2(X[bins], want) < 1e-12
+    assert rel_l2(back, x) < 1e-12

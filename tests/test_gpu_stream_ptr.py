@@ -71,7 +71,7 @@ def test_exec_ptr_errors():
     fw.destroy()
 
 
-@pytest.mark.parametrize("dims", [(256, 64), (64, 8, 4), (8192,), (16, 16)], ids=lambda d: "x".join(map(str, d)))
+@pytest.mark.parametrize("dims", [(256, 64), (64, 8, 4), (8192,), (16, 16), (32768,)], ids=lambda d: "x".join(map(str, d)))
 def test_real_exec_ptr(oracle, torch, dims):
     # topeRealExecPtr on torch arrays in the plan's device layout; the
     # output arrays start as NaN, so every element the result needs must be

@@ -1045,7 +1045,7 @@ int build_real_plan(struct topeFFT *f, long long nx, long long ny, long long nz,
     if (!fw) return TOPE_ERROR_NOT_INITIALIZED;
     if (!real) return TOPE_ERROR_INVALID_ARG;
     if (nx < 2 || ny < 1 || nz < 1 || !is_pow2(nx) || !is_pow2(ny) || !is_pow2(nz)) return TOPE_ERROR_INVALID_SIZE;
-    if (nx / 2 > kMaxFused || ny > kMaxFused || nz > kMaxFused) return TOPE_ERROR_UNSUPPORTED;
+    if (nx / 2 > kMaxAxis || ny > kMaxFused || nz > kMaxFused) return TOPE_ERROR_UNSUPPORTED;
     const long long m = nx / 2, rows = ny * nz;
     if ((m + 1) * rows > (1LL << 33)) return TOPE_ERROR_INVALID_SIZE;
 
@@ -1085,9 +1085,14 @@ int build_real_plan(struct topeFFT *f, long long nx, long long ny, long long nz,
     int cur = 0;
     // X: m-point complex rows of the real array and the half-spectrum step:
     // one fused pass from buffer 0 into buffer 1 where a kernel covers m,
-    // else the row pass in place and the step on its own
+    // else the row transform in place (one pass, or the two four-step passes
+    // above 4096 with buffer 1 as their scratch) and the step on its own.
+    // group_end[g] = passes.size() after group g (rows, step, Y, Z): INVERSE
+    // runs the groups in reverse order, the passes of a group in order.
+    std::vector<size_t> group_end;
     const bool fused = real_rows_supported(m);
     int rc = fused ? (int)TOPE_SUCCESS : pl.axis(m, Geo{1, 0, rows, m, 1}, &cur, 0);
+    group_end.push_back(P->passes.size());
     if (!rc && cur != 0) rc = TOPE_ERROR_UNSUPPORTED;
     if (!rc) {
         void *w = nullptr;
@@ -1110,6 +1115,7 @@ int build_real_plan(struct topeFFT *f, long long nx, long long ny, long long nz,
             P->axis_tw[0].table = stw;
         }
         P->passes.push_back(ps);
+        group_end.push_back(P->passes.size());
         cur = 1;
         // Y: the m + 1 columns kx <= m of every z plane in one pass; its
         // last tile holds only the Nyquist column kx = m (its other lanes
@@ -1124,6 +1130,7 @@ int build_real_plan(struct topeFFT *f, long long nx, long long ny, long long nz,
             if (P->passes.size() != before + 1) return (int)TOPE_ERROR_UNSUPPORTED;
             Pass &ps = P->passes.back();
             ps.a.rot = ps.a32.rot = ps.a.tiles - 1;
+            group_end.push_back(P->passes.size());
             return (int)TOPE_SUCCESS;
         };
         rc = axis2(ny, Geo{nz, pitch * ny, m + 1, 1, pitch}, 1);
@@ -1131,7 +1138,11 @@ int build_real_plan(struct topeFFT *f, long long nx, long long ny, long long nz,
         if (!rc && cur != 1) rc = TOPE_ERROR_UNSUPPORTED;
     }
     if (rc) return fail(rc);
-    for (long long i = (long long)P->passes.size() - 1; i >= 0; --i) P->inv_order.push_back((int)i);
+    for (size_t g = group_end.size(); g-- > 0;) {
+        const size_t begin = g ? group_end[g - 1] : 0;
+        for (size_t i = begin; i < group_end[g]; ++i) P->inv_order.push_back((int)i);
+    }
+    if (P->inv_order.size() != P->passes.size()) return fail(TOPE_ERROR_UNSUPPORTED);
     P->final_buf = 0;
     err = time_stop(pre, s, pre_ns);
     if (err != hipSuccess) return fail(map_err(err));
