@@ -270,7 +270,7 @@ void topePlanExecPtr(struct topeFFT *f, void *plan_impl, void *data, int dir);
 
 /* ---- real-input transforms (no reference counterpart; the reference
  * declares R2C but runs C2C) ---------------------------------------------
- * x (even, x/2 <= 2^24) by y by z (<= 4096 each; 1 for unused axes), powers
+ * x (even, x/2 <= 2^24) by y by z (<= 2^24 each; 1 for unused axes), powers
  * of two, FP64, x fastest.  PlanInit uploads the real array (x*y*z doubles).
  * topeRealExec FORWARD (R2C) transforms the device real array into the half
  * spectrum, (x/2+1)*y*z complex128 in the layout of numpy's rfftn (kx
@@ -297,7 +297,9 @@ void topeRealExecDevice(struct topeFFT *f, struct topePlanReal *t, int dir);
  * `spectrum` dataSize bytes (rows of x/2+1 complex128 at a pitch of `pitch`
  * elements).  Asynchronous on the framework's stream, no host copies.
  * FORWARD writes `spectrum` and may overwrite `real`; INVERSE writes `real`
- * and may overwrite `spectrum`. */
+ * and may overwrite `spectrum`.  Plans with y or z above 4096 use the real
+ * array's buffer as dataSize bytes of scratch and are refused
+ * (TOPE_ERROR_UNSUPPORTED). */
 void topeRealExecPtr(struct topeFFT *f, struct topePlanReal *t, void *real, void *spectrum, int dir);
 void topeRealDestroy(struct topeFFT *f, struct topePlanReal *t);
 

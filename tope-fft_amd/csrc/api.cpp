@@ -108,6 +108,7 @@ struct PlanImpl {
     double real_scale = 1.0;
     size_t real_bytes = 0;
     std::vector<int> inv_order;
+    bool real_scratch0 = false;  // a column pass uses buffer 0 (P->bytes) as scratch
     EventPair run;  // this plan's exec events
     // per-pass timing
     bool timing = false;
@@ -1045,7 +1046,7 @@ int build_real_plan(struct topeFFT *f, long long nx, long long ny, long long nz,
     if (!fw) return TOPE_ERROR_NOT_INITIALIZED;
     if (!real) return TOPE_ERROR_INVALID_ARG;
     if (nx < 2 || ny < 1 || nz < 1 || !is_pow2(nx) || !is_pow2(ny) || !is_pow2(nz)) return TOPE_ERROR_INVALID_SIZE;
-    if (nx / 2 > kMaxAxis || ny > kMaxFused || nz > kMaxFused) return TOPE_ERROR_UNSUPPORTED;
+    if (nx / 2 > kMaxAxis || ny > kMaxAxis || nz > kMaxAxis) return TOPE_ERROR_UNSUPPORTED;
     const long long m = nx / 2, rows = ny * nz;
     if ((m + 1) * rows > (1LL << 33)) return TOPE_ERROR_INVALID_SIZE;
 
@@ -1121,15 +1122,18 @@ int build_real_plan(struct topeFFT *f, long long nx, long long ny, long long nz,
         // last tile holds only the Nyquist column kx = m (its other lanes
         // are masked and touch no memory) and is rotated to run first, so it
         // overlaps the full tiles instead of trailing them.  Z likewise per
-        // y row.
+        // y row.  Axes above 4096 run the four-step, with buffer 0 (the real
+        // array's, P->bytes long) as the scratch of its first pass.
         auto axis2 = [&](long long n, Geo g, int id) {
             if (n == 1) return (int)TOPE_SUCCESS;
             const size_t before = P->passes.size();
             int r = pl.axis(n, g, &cur, id);
             if (r) return r;
-            if (P->passes.size() != before + 1) return (int)TOPE_ERROR_UNSUPPORTED;
-            Pass &ps = P->passes.back();
-            ps.a.rot = ps.a32.rot = ps.a.tiles - 1;
+            for (size_t i = before; i < P->passes.size(); ++i) {
+                Pass &ps = P->passes[i];
+                if (ps.kind == PASS_AXIS) ps.a.rot = ps.a32.rot = ps.a.tiles - 1;
+                if (ps.in_buf == 0 || ps.out_buf == 0) P->real_scratch0 = true;
+            }
             group_end.push_back(P->passes.size());
             return (int)TOPE_SUCCESS;
         };
@@ -1807,6 +1811,12 @@ void topeRealExecPtr(struct topeFFT *f, struct topePlanReal *t, void *real, void
     }
     if (!real || !spectrum || (dir != FORWARD && dir != INVERSE)) {
         f->error = TOPE_ERROR_INVALID_ARG;
+        return;
+    }
+    if (P->real_scratch0) {
+        // y or z above 4096: the four-step scratch needs dataSize bytes of
+        // buffer 0, more than a caller's real array holds
+        f->error = TOPE_ERROR_UNSUPPORTED;
         return;
     }
     // the plan's passes run on the caller's arrays in place of its own
