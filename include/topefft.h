@@ -270,8 +270,10 @@ void topePlanExecPtr(struct topeFFT *f, void *plan_impl, void *data, int dir);
 
 /* ---- real-input transforms (no reference counterpart; the reference
  * declares R2C but runs C2C) ---------------------------------------------
- * x (even, x/2 <= 2^24) by y by z (<= 2^24 each; 1 for unused axes), powers
- * of two, FP64, x fastest.  PlanInit uploads the real array (x*y*z doubles).
+ * x (even) by y by z (1 for unused axes), FP64, x fastest; x/2, y and z
+ * each a power of two up to 2^24 or 7-smooth (factors 2, 3, 5, 7) up to 4096
+ * (others: TOPE_ERROR_UNSUPPORTED, odd x: TOPE_ERROR_INVALID_SIZE).
+ * PlanInit uploads the real array (x*y*z doubles).
  * topeRealExec FORWARD (R2C) transforms the device real array into the half
  * spectrum, (x/2+1)*y*z complex128 in the layout of numpy's rfftn (kx
  * fastest), and downloads it into host (2*(x/2+1)*y*z doubles; on the

@@ -2,7 +2,8 @@
 oracle's full complex transform (sliced to kx <= nx/2), C2R by round trip and
 against the oracle's inverse, rows above 8192 and columns above 4096
 through the four-step (full comparison up to 32768 points per axis, sampled
-bins at 2^22), closed forms, and the size limits.  Bar:
+bins at 2^22), 7-smooth axes through the mixed-radix pass, closed forms, and
+the size limits.  Bar:
 rel-L2 <= 1e-12, as for the complex FP64 path."""
 import numpy as np
 import pytest
@@ -15,7 +16,7 @@ pytestmark = pytest.mark.gpu
 # x above 8192: the row transform runs as the four-step (two passes) and the
 # half-spectrum step on its own
 SHAPES = [(2,), (4,), (16,), (256,), (4096,), (8192,), (16384,), (32768,), (16384, 4), (16384, 2, 2),
-          (16, 8192), (4, 2, 8192), (8, 4), (64, 32), (256, 256), (2, 8),
+          (16, 8192), (4, 2, 8192), (200,), (24, 10), (30, 6, 10), (2000, 3), (8, 4), (64, 32), (256, 256), (2, 8),
           (16, 8, 4), (32, 32, 32), (2, 4, 8), (128, 2, 16)]
 
 
@@ -92,8 +93,12 @@ def test_r2c_closed_forms():
 def test_real_plan_errors():
     fw = topefft.Framework()
     with pytest.raises(topefft.TopeError) as e:
-        topefft.RealPlan(fw, (6,), np.zeros(6))
+        topefft.RealPlan(fw, (7,), np.zeros(7))
     assert e.value.code == topefft.TOPE_ERROR_INVALID_SIZE
+    # x/2 = 11: neither a power of two nor 7-smooth
+    with pytest.raises(topefft.TopeError) as e:
+        topefft.RealPlan(fw, (22,), np.zeros(22))
+    assert e.value.code == topefft.TOPE_ERROR_UNSUPPORTED
     # y above 4096: the column four-step uses the real array's buffer as
     # scratch at the spectrum's size, so caller arrays are refused
     p = topefft.RealPlan(fw, (16, 8192), np.zeros(16 * 8192))

@@ -1045,8 +1045,11 @@ int build_real_plan(struct topeFFT *f, long long nx, long long ny, long long nz,
     FrameworkImpl *fw = fw_of(f);
     if (!fw) return TOPE_ERROR_NOT_INITIALIZED;
     if (!real) return TOPE_ERROR_INVALID_ARG;
-    if (nx < 2 || ny < 1 || nz < 1 || !is_pow2(nx) || !is_pow2(ny) || !is_pow2(nz)) return TOPE_ERROR_INVALID_SIZE;
-    if (nx / 2 > kMaxAxis || ny > kMaxAxis || nz > kMaxAxis) return TOPE_ERROR_UNSUPPORTED;
+    if (nx < 2 || (nx & 1) || ny < 1 || nz < 1) return TOPE_ERROR_INVALID_SIZE;
+    // each axis (x/2 for the rows) a power of two up to 2^24, or 7-smooth up
+    // to 4096 (one mixed-radix pass)
+    auto covered = [](long long n) { return is_pow2(n) ? n <= kMaxAxis : mixed_supported(n); };
+    if (!covered(nx / 2) || !covered(ny) || !covered(nz)) return TOPE_ERROR_UNSUPPORTED;
     const long long m = nx / 2, rows = ny * nz;
     if ((m + 1) * rows > (1LL << 33)) return TOPE_ERROR_INVALID_SIZE;
 
@@ -1092,7 +1095,9 @@ int build_real_plan(struct topeFFT *f, long long nx, long long ny, long long nz,
     // runs the groups in reverse order, the passes of a group in order.
     std::vector<size_t> group_end;
     const bool fused = real_rows_supported(m);
-    int rc = fused ? (int)TOPE_SUCCESS : pl.axis(m, Geo{1, 0, rows, m, 1}, &cur, 0);
+    const Geo rows_geo{1, 0, rows, m, 1};
+    int rc = fused ? (int)TOPE_SUCCESS
+                   : (is_pow2(m) ? pl.axis(m, rows_geo, &cur, 0) : pl.mixed_axis(m, rows_geo, cur, 0));
     group_end.push_back(P->passes.size());
     if (!rc && cur != 0) rc = TOPE_ERROR_UNSUPPORTED;
     if (!rc) {
@@ -1127,7 +1132,7 @@ int build_real_plan(struct topeFFT *f, long long nx, long long ny, long long nz,
         auto axis2 = [&](long long n, Geo g, int id) {
             if (n == 1) return (int)TOPE_SUCCESS;
             const size_t before = P->passes.size();
-            int r = pl.axis(n, g, &cur, id);
+            int r = is_pow2(n) ? pl.axis(n, g, &cur, id) : pl.mixed_axis(n, g, cur, id);
             if (r) return r;
             for (size_t i = before; i < P->passes.size(); ++i) {
                 Pass &ps = P->passes[i];
