@@ -201,7 +201,9 @@ const char *topePlanPassName(void *plan_impl, int pass);
  *   TOPE_MODE_MFMA_FP16X3 2 fp16 parts, 3 products (~FP32 accuracy), each
  *                         transform scaled by a power of two into fp16 range
  * MFMA modes take n = 256, 512, 1024, 2048 or 4096.  Twiddles and DFT-matrix constants are
- * generated in FP64 on the GPU and rounded once; INVERSE scales by 1/n. */
+ * generated in FP64 on the GPU and rounded once; INVERSE scales by 1/n.
+ * FP32 plans take powers of two; FP64 plans also take 7-smooth n up to 4096
+ * (mixed-radix pass) and other n up to 2048 (Bluestein). */
 #define TOPE_PREC_F64 0
 #define TOPE_PREC_F32 1
 #define TOPE_MODE_VALU 0

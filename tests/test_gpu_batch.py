@@ -32,7 +32,8 @@ def run_batch(n, count, x, prec, mode, direction):
         fw.destroy()
 
 
-@pytest.mark.parametrize("n", [2, 16, 256, 1024, 4096, 8192])
+# powers of two; 7-smooth lengths (mixed-radix pass); others (Bluestein)
+@pytest.mark.parametrize("n", [2, 16, 256, 1024, 4096, 8192, 3, 12, 100, 1000, 4000, 11, 1009, 2048 - 1])
 def test_batch_fp64(oracle, n):
     count = 12
     x = batch_input(n, count, 100 + n)
@@ -128,4 +129,12 @@ def test_batch_rejects_unsupported():
     assert e.value.code == topefft.TOPE_ERROR_UNSUPPORTED
     with pytest.raises(topefft.TopeError):
         topefft.BatchPlan(fw, 256, 2, np.zeros(512, complex), topefft.PREC_F64, topefft.MODE_MFMA_FP32)
+    # lengths other than powers of two: FP64 only
+    with pytest.raises(topefft.TopeError) as e:
+        topefft.BatchPlan(fw, 100, 2, np.zeros(200, np.complex64), topefft.PREC_F32, topefft.MODE_VALU)
+    assert e.value.code == topefft.TOPE_ERROR_UNSUPPORTED
+    # Bluestein rows of M = 2^k >= 2n - 1 up to 4096
+    with pytest.raises(topefft.TopeError) as e:
+        topefft.BatchPlan(fw, 2053, 2, np.zeros(4106, complex), topefft.PREC_F64, topefft.MODE_VALU)
+    assert e.value.code == topefft.TOPE_ERROR_INVALID_SIZE
     fw.destroy()

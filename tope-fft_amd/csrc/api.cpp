@@ -852,7 +852,8 @@ int build_plan(struct topeFFT *f, int shape, long long nx, long long ny, long lo
     if (total > (1LL << 34)) return TOPE_ERROR_INVALID_SIZE;
     if ((shape == 2 || shape == 3) && prec != PREC_F64 && (!is_pow2(nx) || !is_pow2(ny) || !is_pow2(nz)))
         return TOPE_ERROR_UNSUPPORTED;
-    if (shape == SHAPE_BATCH && !is_pow2(nx)) return TOPE_ERROR_INVALID_SIZE;
+    // batches of other lengths: FP64 on the vector ALUs (mixed radix or Bluestein)
+    if (shape == SHAPE_BATCH && !is_pow2(nx) && (prec != PREC_F64 || mode != MP_VALU)) return TOPE_ERROR_UNSUPPORTED;
     if (prec != PREC_F64 && prec != PREC_F32) return TOPE_ERROR_INVALID_ARG;
     if (mode != MP_VALU && (prec != PREC_F32 || (shape != SHAPE_BATCH && shape != 2 && shape != 3)))
         return TOPE_ERROR_UNSUPPORTED;
@@ -953,7 +954,7 @@ int build_plan(struct topeFFT *f, int shape, long long nx, long long ny, long lo
         P->axis_tw[0].e = 16;
         P->axis_tw[0].table = tw;
     } else if (shape == SHAPE_BATCH) {
-        rc = pl.axis(nx, Geo{1, 0, ny, nx, 1}, &cur, 0);
+        rc = any_axis(pl, nx, Geo{1, 0, ny, nx, 1}, &cur, 0);
     } else if (nonpow2 && !(shape == 1 && !chirpz && !mixed_supported(nx))) {
         rc = any_axis(pl, nx, Geo{1, 0, ny * nz, nx, 1}, &cur, 0);
         if (!rc && shape >= 2) rc = any_axis(pl, ny, Geo{nz, nx * ny, nx, 1, nx}, &cur, 1);
