@@ -202,8 +202,8 @@ const char *topePlanPassName(void *plan_impl, int pass);
  *                         transform scaled by a power of two into fp16 range
  * MFMA modes take n = 256, 512, 1024, 2048 or 4096.  Twiddles and DFT-matrix constants are
  * generated in FP64 on the GPU and rounded once; INVERSE scales by 1/n.
- * FP32 plans take powers of two; FP64 plans also take 7-smooth n up to 4096
- * (mixed-radix pass) and other n up to 2048 (Bluestein). */
+ * Both precisions take powers of two and 7-smooth n up to 4096 (mixed-radix
+ * pass, VALU mode); FP64 plans also take other n up to 2048 (Bluestein). */
 #define TOPE_PREC_F64 0
 #define TOPE_PREC_F32 1
 #define TOPE_MODE_VALU 0
@@ -234,7 +234,7 @@ void topeBatchDestroy(struct topeFFT *f, struct topePlanBatch *t);
 /* 2D/3D plans in a chosen precision: TOPE_PREC_F64 (complex128, host
  * double*, identical to tope2DPlanInit/tope3DPlanInit) or TOPE_PREC_F32
  * (complex64 storage, host float*, FP32 butterflies with FP64-generated
- * twiddles).  Exec/ExecDevice/Destroy as for the reference plans; the Ex
+ * twiddles; axes powers of two or 7-smooth up to 4096).  Exec/ExecDevice/Destroy as for the reference plans; the Ex
  * Exec variants take the host buffer as void*. */
 void tope2DPlanInitEx(struct topeFFT *f, struct topePlan2D *t, int x, int y, int type, int precision,
                       const void *host);

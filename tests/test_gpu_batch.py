@@ -121,6 +121,18 @@ def test_batch_round_trip_mfma():
     assert rel_l2(back.astype(np.complex128), x.astype(np.complex128)) < 2e-6
 
 
+@pytest.mark.parametrize("n", [12, 100, 1000, 3000])
+def test_batch_complex64_mixed_radix(oracle, n):
+    # complex64 batch of 7-smooth lengths: the mixed-radix pass in FP32
+    count = 10
+    x = batch_input(n, count, 300 + n).astype(np.complex64)
+    x64 = x.astype(np.complex128)
+    for d in (topefft.FORWARD, topefft.INVERSE):
+        got = run_batch(n, count, x, topefft.PREC_F32, topefft.MODE_VALU, d).reshape(count, n).astype(np.complex128)
+        for r in (0, 4, count - 1):
+            assert rel_l2(got[r], oracle.dft(x64[r * n:(r + 1) * n], (n,), d)) < 1e-6, (n, d, r)
+
+
 def test_batch_rejects_unsupported():
     fw = topefft.Framework()
     x = np.zeros(128 * 2, np.complex64)
@@ -129,9 +141,13 @@ def test_batch_rejects_unsupported():
     assert e.value.code == topefft.TOPE_ERROR_UNSUPPORTED
     with pytest.raises(topefft.TopeError):
         topefft.BatchPlan(fw, 256, 2, np.zeros(512, complex), topefft.PREC_F64, topefft.MODE_MFMA_FP32)
-    # lengths other than powers of two: FP64 only
+    # complex64 batches: powers of two and 7-smooth lengths on the vector
+    # ALUs only
     with pytest.raises(topefft.TopeError) as e:
-        topefft.BatchPlan(fw, 100, 2, np.zeros(200, np.complex64), topefft.PREC_F32, topefft.MODE_VALU)
+        topefft.BatchPlan(fw, 11, 2, np.zeros(22, np.complex64), topefft.PREC_F32, topefft.MODE_VALU)
+    assert e.value.code == topefft.TOPE_ERROR_UNSUPPORTED
+    with pytest.raises(topefft.TopeError) as e:
+        topefft.BatchPlan(fw, 100, 2, np.zeros(200, np.complex64), topefft.PREC_F32, topefft.MODE_MFMA_FP16X3)
     assert e.value.code == topefft.TOPE_ERROR_UNSUPPORTED
     # Bluestein rows of M = 2^k >= 2n - 1 up to 4096
     with pytest.raises(topefft.TopeError) as e:

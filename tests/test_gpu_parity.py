@@ -369,10 +369,12 @@ def test_2d_dif2_schedule_small(oracle, monkeypatch, dims):
     assert rel_l2(back, x) < TOL
 
 
-@pytest.mark.parametrize("dims", [(16, 16), (64, 32), (8, 16, 4), (32, 32, 32), (64, 4, 8)],
+@pytest.mark.parametrize("dims", [(16, 16), (64, 32), (8, 16, 4), (32, 32, 32), (64, 4, 8),
+                                  (12, 10), (100, 6), (15, 7, 9), (1000, 3), (3000, 2)],
                          ids=lambda d: "x".join(map(str, d)))
 def test_nd_complex64(oracle, dims):
-    # complex64 storage plans (tope2DPlanInitEx / tope3DPlanInitEx)
+    # complex64 storage plans (tope2DPlanInitEx / tope3DPlanInitEx); 7-smooth
+    # axes run the mixed-radix pass in FP32
     x = seeded(dims, salt=9).astype(np.complex64)
     x64 = x.astype(np.complex128)
     fw = topefft.Framework()
@@ -426,3 +428,17 @@ def test_3d_256_row_tiles_complex64_against_fp64():
     assert names32[0] == "axis_float_n256_e16_tb4_row", names32
     assert rel_l2(back, x) < TOL
     assert rel_l2(X32, X) < 1e-6
+
+
+def test_complex64_rejects_uncovered_axes():
+    fw = topefft.Framework()
+    # 11 is neither a power of two nor 7-smooth: FP64 only (Bluestein)
+    with pytest.raises(topefft.TopeError) as e:
+        topefft.Plan(fw, (11, 4), np.zeros(44, np.complex64), precision=topefft.PREC_F32)
+    assert e.value.code == topefft.TOPE_ERROR_UNSUPPORTED
+    # the matrix-core modes take powers of two only
+    with pytest.raises(topefft.TopeError) as e:
+        topefft.Plan(fw, (12, 256), np.zeros(12 * 256, np.complex64), precision=topefft.PREC_F32,
+                     mode=topefft.MODE_MFMA_FP16X3)
+    assert e.value.code == topefft.TOPE_ERROR_UNSUPPORTED
+    fw.destroy()

@@ -505,7 +505,7 @@ class Planner {
         Pass ps;
         ps.kind = PASS_MIXED;
         ps.mx = mixed_plan(n, g.O, g.os, g.C, g.cs, g.is);
-        ps.mx.roots = static_cast<const cpx<double> *>(roots);
+        ps.mx.roots = roots;
         ps.grid = mixed_grid(ps.mx);
         ps.in_buf = ps.out_buf = cur;
         ps.name = "mixed_radix_f64";
@@ -769,7 +769,7 @@ int launch_pass(PlanImpl *P, size_t i, hipStream_t s, int dir)
     } else if (ps.kind == PASS_MIXED) {
         MixedArgs m = ps.mx;
         m.scale = ps.last ? sc : 1.0;
-        hipError_t err = launch_mixed(ib, ob, m, sign, s);
+        hipError_t err = launch_mixed(P->prec, ib, ob, m, sign, s);
         if (err != hipSuccess) return map_err(err);
     } else if (ps.kind == PASS_CHIRP) {
         // 1/M undoes the unnormalised inner inverse FFT; the plan's 1/N of
@@ -850,10 +850,15 @@ int build_plan(struct topeFFT *f, int shape, long long nx, long long ny, long lo
     if (nx < 1 || ny < 1 || nz < 1) return TOPE_ERROR_INVALID_SIZE;
     const long long total = nx * ny * nz;
     if (total > (1LL << 34)) return TOPE_ERROR_INVALID_SIZE;
-    if ((shape == 2 || shape == 3) && prec != PREC_F64 && (!is_pow2(nx) || !is_pow2(ny) || !is_pow2(nz)))
-        return TOPE_ERROR_UNSUPPORTED;
-    // batches of other lengths: FP64 on the vector ALUs (mixed radix or Bluestein)
-    if (shape == SHAPE_BATCH && !is_pow2(nx) && (prec != PREC_F64 || mode != MP_VALU)) return TOPE_ERROR_UNSUPPORTED;
+    // complex64 axes: powers of two, or 7-smooth up to 4096 (mixed-radix
+    // pass) on the vector ALUs; FP64 also takes Bluestein for the others
+    // (a batch transforms only its rows: ny is the count)
+    auto f32_axis = [](long long n) { return is_pow2(n) || mixed_supported(n); };
+    const bool batch = shape == SHAPE_BATCH;
+    const bool f32_ok = f32_axis(nx) && (batch || (f32_axis(ny) && f32_axis(nz)));
+    const bool all_pow2 = is_pow2(nx) && (batch || (is_pow2(ny) && is_pow2(nz)));
+    if ((shape == 2 || shape == 3 || batch) && prec != PREC_F64 && !f32_ok) return TOPE_ERROR_UNSUPPORTED;
+    if (mode != MP_VALU && !all_pow2) return TOPE_ERROR_UNSUPPORTED;
     if (prec != PREC_F64 && prec != PREC_F32) return TOPE_ERROR_INVALID_ARG;
     if (mode != MP_VALU && (prec != PREC_F32 || (shape != SHAPE_BATCH && shape != 2 && shape != 3)))
         return TOPE_ERROR_UNSUPPORTED;

@@ -91,13 +91,13 @@ struct MixedArgs {
     long long O, os, C, cs, is, lines;
     int n, tb, nst;
     int radix[16];
-    const cpx<double> *roots;
+    const void *roots;  // W_n^i, i < n, in the pass's precision (forward sign)
     double scale;
 };
 bool mixed_supported(long long n);
 MixedArgs mixed_plan(long long n, long long O, long long os, long long C, long long cs, long long is);
 unsigned mixed_grid(const MixedArgs &a);
-hipError_t launch_mixed(const void *in, void *out, const MixedArgs &a, int sign, hipStream_t st);
+hipError_t launch_mixed(int prec, const void *in, void *out, const MixedArgs &a, int sign, hipStream_t st);
 struct ChirpGeo {
     long long O, os, C, cs, is;
 };

@@ -1,4 +1,5 @@
-// Mixed-radix axis pass for lengths n = 2^a 3^b 5^c 7^d <= 4096 (FP64).
+// Mixed-radix axis pass for lengths n = 2^a 3^b 5^c 7^d <= 4096 (FP64, and
+// complex64 storage with FP32 arithmetic and the table rounded once).
 // The reference runs such 1D sizes as an O(n^2) DFT (src/fft1d.c:148-232,
 // src/kernels1D.cl:549-573) and rejects them in 2D/3D (src/fft2d.c:355-362,
 // src/fft3d.c:522-533).
@@ -62,21 +63,21 @@ __device__ __forceinline__ long long divmod64(long long a, const Div64 &d, long 
     return q;
 }
 
-template <int SIGN>
-__device__ __forceinline__ cpx<double> root(const cpx<double> *__restrict__ w, long long i)
+template <int SIGN, typename R>
+__device__ __forceinline__ cpx<R> root(const cpx<R> *__restrict__ w, long long i)
 {
-    const cpx<double> v = w[i];
-    return SIGN < 0 ? v : cpx<double>{v.x, -v.y};
+    const cpx<R> v = w[i];
+    return SIGN < 0 ? v : cpx<R>{v.x, -v.y};
 }
 
 // Odd prime radix r by the conjugate-pair form: with a_j = x_j + x_{r-j},
 // b_j = x_j - x_{r-j} and W_r^{jk} = c + i s,
 //   y_k = x_0 + sum_{j=1}^{(r-1)/2} (a_j c + i b_j s).
-template <int r, int SIGN>
-__device__ __forceinline__ void dft_odd(cpx<double> (&x)[r], const cpx<double> *__restrict__ w, long long step)
+template <int r, int SIGN, typename R>
+__device__ __forceinline__ void dft_odd(cpx<R> (&x)[r], const cpx<R> *__restrict__ w, long long step)
 {
     constexpr int h = (r - 1) / 2;
-    cpx<double> a[h], b[h], wr[r];
+    cpx<R> a[h], b[h], wr[r];
 #pragma unroll
     for (int j = 1; j <= h; ++j) {
         a[j - 1] = x[j] + x[r - j];
@@ -84,16 +85,16 @@ __device__ __forceinline__ void dft_odd(cpx<double> (&x)[r], const cpx<double> *
     }
 #pragma unroll
     for (int m = 1; m < r; ++m) wr[m] = root<SIGN>(w, m * step);
-    cpx<double> y[r];
+    cpx<R> y[r];
     y[0] = x[0];
 #pragma unroll
     for (int j = 0; j < h; ++j) y[0] = y[0] + a[j];
 #pragma unroll
     for (int k = 1; k < r; ++k) {
-        cpx<double> acc = x[0];
+        cpx<R> acc = x[0];
 #pragma unroll
         for (int j = 1; j <= h; ++j) {
-            const cpx<double> c = wr[(j * k) % r];
+            const cpx<R> c = wr[(j * k) % r];
             acc.x += a[j - 1].x * c.x - b[j - 1].y * c.y;
             acc.y += a[j - 1].y * c.x + b[j - 1].x * c.y;
         }
@@ -108,9 +109,9 @@ __device__ __forceinline__ void dft_odd(cpx<double> (&x)[r], const cpx<double> *
 __device__ __forceinline__ int lslot(int i) { return i + (i >> 4); }
 
 // One stage of compile-time radix r over the block's lines, LDS to LDS.
-template <int r, int SIGN, int NT>
-__device__ __forceinline__ void mixed_stage(const cpx<double> *src, cpx<double> *dst, int n, int tb, int Ns,
-                                            const cpx<double> *__restrict__ roots)
+template <typename R, int r, int SIGN, int NT>
+__device__ __forceinline__ void mixed_stage(const cpx<R> *src, cpx<R> *dst, int n, int tb, int Ns,
+                                            const cpx<R> *__restrict__ roots)
 {
     const int nb = n / r;  // butterflies per line
     const long long tws = (long long)n / ((long long)Ns * r);
@@ -119,7 +120,7 @@ __device__ __forceinline__ void mixed_stage(const cpx<double> *src, cpx<double> 
         int j, kk;
         const int t = divmod(bi, dnb, j);
         const int jn = divmod(j, dns, kk);  // j = jn Ns + kk
-        cpx<double> x[r];
+        cpx<R> x[r];
         const int si = t * n + j;
 #pragma unroll
         for (int m = 0; m < r; ++m) x[m] = src[lslot(si + m * nb)];
@@ -140,11 +141,12 @@ __device__ __forceinline__ void mixed_stage(const cpx<double> *src, cpx<double> 
 // thread keep the latency of the stage loop covered; 256 threads measured
 // 10-12 % slower at 100^3 and 125^3), 1024 for the single-line 4096-element
 // blocks that own a CU.
-template <int SIGN, int CAP, int NT>
-__global__ void __launch_bounds__(NT) mixed_kernel(const cpx<double> *in, cpx<double> *out, const MixedArgs a)
+template <typename R, int SIGN, int CAP, int NT>
+__global__ void __launch_bounds__(NT) mixed_kernel(const cpx<R> *in, cpx<R> *out, const MixedArgs a)
 {
     constexpr int SLOTS = CAP + CAP / 16;
-    __shared__ cpx<double> lds[2 * SLOTS];
+    __shared__ cpx<R> lds[2 * SLOTS];
+    const cpx<R> *roots = static_cast<const cpx<R> *>(a.roots);
     const int n = a.n, tb = a.tb;
     const int span_elems = tb * n;
     const bool cfast = a.is != 1;  // strided axis: neighbouring lanes take neighbouring lines
@@ -169,25 +171,25 @@ __global__ void __launch_bounds__(NT) mixed_kernel(const cpx<double> *in, cpx<do
         int t, k;
         split(idx, t, k);
         const long long l = l0 + t;
-        lds[lslot(t * n + k)] = l < a.lines ? in[gaddr(l, k)] : cpx<double>{0, 0};
+        lds[lslot(t * n + k)] = l < a.lines ? in[gaddr(l, k)] : cpx<R>{0, 0};
     }
     __syncthreads();
 
-    cpx<double> *src = lds, *dst = lds + SLOTS;
+    cpx<R> *src = lds, *dst = lds + SLOTS;
     int Ns = 1;
     for (int s = 0; s < a.nst; ++s) {
         const int r = a.radix[s];
         switch (r) {
-        case 2: mixed_stage<2, SIGN, NT>(src, dst, n, tb, Ns, a.roots); break;
-        case 3: mixed_stage<3, SIGN, NT>(src, dst, n, tb, Ns, a.roots); break;
-        case 4: mixed_stage<4, SIGN, NT>(src, dst, n, tb, Ns, a.roots); break;
-        case 5: mixed_stage<5, SIGN, NT>(src, dst, n, tb, Ns, a.roots); break;
-        case 7: mixed_stage<7, SIGN, NT>(src, dst, n, tb, Ns, a.roots); break;
-        case 8: mixed_stage<8, SIGN, NT>(src, dst, n, tb, Ns, a.roots); break;
-        default: mixed_stage<16, SIGN, NT>(src, dst, n, tb, Ns, a.roots); break;
+        case 2: mixed_stage<R, 2, SIGN, NT>(src, dst, n, tb, Ns, roots); break;
+        case 3: mixed_stage<R, 3, SIGN, NT>(src, dst, n, tb, Ns, roots); break;
+        case 4: mixed_stage<R, 4, SIGN, NT>(src, dst, n, tb, Ns, roots); break;
+        case 5: mixed_stage<R, 5, SIGN, NT>(src, dst, n, tb, Ns, roots); break;
+        case 7: mixed_stage<R, 7, SIGN, NT>(src, dst, n, tb, Ns, roots); break;
+        case 8: mixed_stage<R, 8, SIGN, NT>(src, dst, n, tb, Ns, roots); break;
+        default: mixed_stage<R, 16, SIGN, NT>(src, dst, n, tb, Ns, roots); break;
         }
         __syncthreads();
-        cpx<double> *tmp = src;
+        cpx<R> *tmp = src;
         src = dst;
         dst = tmp;
         Ns *= r;
@@ -198,8 +200,8 @@ __global__ void __launch_bounds__(NT) mixed_kernel(const cpx<double> *in, cpx<do
         split(idx, t, k);
         const long long l = l0 + t;
         if (l < a.lines) {
-            const cpx<double> v = src[lslot(t * n + k)];
-            out[gaddr(l, k)] = cpx<double>{v.x * a.scale, v.y * a.scale};
+            const cpx<R> v = src[lslot(t * n + k)];
+            out[gaddr(l, k)] = cpx<R>{v.x * (R)a.scale, v.y * (R)a.scale};
         }
     }
 }
@@ -242,21 +244,28 @@ MixedArgs mixed_plan(long long n, long long O, long long os, long long C, long l
 
 unsigned mixed_grid(const MixedArgs &a) { return (unsigned)((a.lines + a.tb - 1) / a.tb); }
 
-hipError_t launch_mixed(const void *in, void *out, const MixedArgs &a, int sign, hipStream_t st)
+template <typename R>
+void launch_mixed_t(const void *in, void *out, const MixedArgs &a, int sign, const dim3 g, hipStream_t st)
+{
+    const cpx<R> *i = static_cast<const cpx<R> *>(in);
+    cpx<R> *o = static_cast<cpx<R> *>(out);
+    if ((long long)a.tb * a.n <= 2048) {
+        if (sign < 0) hipLaunchKernelGGL((mixed_kernel<R, -1, 2048, 512>), g, dim3(512), 0, st, i, o, a);
+        else hipLaunchKernelGGL((mixed_kernel<R, 1, 2048, 512>), g, dim3(512), 0, st, i, o, a);
+    } else {
+        if (sign < 0) hipLaunchKernelGGL((mixed_kernel<R, -1, MIXED_MAX_N, 1024>), g, dim3(1024), 0, st, i, o, a);
+        else hipLaunchKernelGGL((mixed_kernel<R, 1, MIXED_MAX_N, 1024>), g, dim3(1024), 0, st, i, o, a);
+    }
+}
+
+hipError_t launch_mixed(int prec, const void *in, void *out, const MixedArgs &a, int sign, hipStream_t st)
 {
     const long long grid = (a.lines + a.tb - 1) / a.tb;
     if (grid > 0x7fffffffLL || a.nst < 1 || a.nst > 16 || a.tb < 1 || (long long)a.tb * a.n > MIXED_MAX_N)
         return hipErrorInvalidValue;
-    const cpx<double> *i = static_cast<const cpx<double> *>(in);
-    cpx<double> *o = static_cast<cpx<double> *>(out);
     const dim3 g((unsigned)grid);
-    if ((long long)a.tb * a.n <= 2048) {
-        if (sign < 0) hipLaunchKernelGGL((mixed_kernel<-1, 2048, 512>), g, dim3(512), 0, st, i, o, a);
-        else hipLaunchKernelGGL((mixed_kernel<1, 2048, 512>), g, dim3(512), 0, st, i, o, a);
-    } else {
-        if (sign < 0) hipLaunchKernelGGL((mixed_kernel<-1, MIXED_MAX_N, 1024>), g, dim3(1024), 0, st, i, o, a);
-        else hipLaunchKernelGGL((mixed_kernel<1, MIXED_MAX_N, 1024>), g, dim3(1024), 0, st, i, o, a);
-    }
+    if (prec == PREC_F64) launch_mixed_t<double>(in, out, a, sign, g, st);
+    else launch_mixed_t<float>(in, out, a, sign, g, st);
     return hipGetLastError();
 }
 
