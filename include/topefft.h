@@ -285,7 +285,8 @@ void topePlanExecPtr(struct topeFFT *f, void *plan_impl, void *data, int dir);
  * real array and INVERSE the half spectrum (in-place pipeline). */
 struct topePlanReal {
     int x, y, z, dim;
-    int pitch, reserved; /* device half-spectrum row pitch (elements) */
+    int pitch;       /* device half-spectrum row pitch (elements) */
+    int precision;   /* TOPE_PREC_F64 or TOPE_PREC_F32 (topeRealPlanInitEx) */
     void *data;      /* device half spectrum */
     void *real;      /* device real array */
     size_t dataSize; /* bytes of data */
@@ -296,6 +297,13 @@ struct topePlanReal {
 void topeRealPlanInit(struct topeFFT *f, struct topePlanReal *t, int x, int y, int z, const double *real);
 void topeRealExec(struct topeFFT *f, struct topePlanReal *t, double *host, int dir);
 void topeRealExecDevice(struct topeFFT *f, struct topePlanReal *t, int dir);
+/* Precision variant: TOPE_PREC_F64 (as above) or TOPE_PREC_F32 (float real
+ * array, complex64 half spectrum, FP32 arithmetic with FP64-generated
+ * tables rounded once; x/2, y, z powers of two or 7-smooth up to 4096).
+ * ExecEx downloads into `host` as Exec does, in the plan's precision. */
+void topeRealPlanInitEx(struct topeFFT *f, struct topePlanReal *t, int x, int y, int z, int precision,
+                        const void *real);
+void topeRealExecEx(struct topeFFT *f, struct topePlanReal *t, void *host, int dir);
 /* topeRealExecPtr: the same transform on caller-owned device arrays, in
  * the plan's device layout: `real` holds x*y*z doubles (realSize bytes),
  * `spectrum` dataSize bytes (rows of x/2+1 complex128 at a pitch of `pitch`

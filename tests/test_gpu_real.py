@@ -2,8 +2,8 @@
 oracle's full complex transform (sliced to kx <= nx/2), C2R by round trip and
 against the oracle's inverse, rows above 8192 and columns above 4096
 through the four-step (full comparison up to 32768 points per axis, sampled
-bins at 2^22), 7-smooth axes through the mixed-radix pass, closed forms, and
-the size limits.  Bar:
+bins at 2^22), 7-smooth axes through the mixed-radix pass, complex64
+storage (1e-6), closed forms, and the size limits.  Bar:
 rel-L2 <= 1e-12, as for the complex FP64 path."""
 import numpy as np
 import pytest
@@ -127,3 +127,34 @@ def test_r2c_large_four_step_bins(oracle):
     want = oracle.dft_bins(x.astype(np.complex128), (n,), bins, topefft.FORWARD)
     assert rel_l2(X[bins], want) < 1e-12
     assert rel_l2(back, x) < 1e-12
+
+
+C64_SHAPES = [(256,), (4096,), (8192,), (64, 32), (256, 256), (16, 8, 4), (2, 8), (200,), (24, 10), (30, 6, 10)]
+
+
+@pytest.mark.parametrize("dims", C64_SHAPES, ids=lambda d: "x".join(map(str, d)))
+def test_r2c_c2r_complex64(oracle, dims):
+    # complex64 storage (topeRealPlanInitEx with TOPE_PREC_F32): FP32
+    # arithmetic, tables generated in FP64 and rounded once; bar 1e-6
+    # against the FP64 oracle
+    x = real_input(dims, 90 + int(np.prod(dims)) % 17).astype(np.float32)
+    fw = topefft.Framework()
+    p = topefft.RealPlan(fw, dims, x, precision=topefft.PREC_F32)
+    assert p.t.precision == topefft.PREC_F32 and p.t.realSize == x.size * 4
+    X = p.forward()
+    back = p.inverse()
+    p.destroy()
+    fw.destroy()
+    assert X.dtype == np.complex64 and back.dtype == np.float32
+    want = half(oracle.dft(x.astype(np.complex128), dims, topefft.FORWARD), dims)
+    assert rel_l2(X.astype(np.complex128), want) < 1e-6
+    assert rel_l2(back.astype(np.float64), x.astype(np.float64)) < 1e-6
+
+
+def test_real_complex64_limits():
+    fw = topefft.Framework()
+    # complex64 rows stop at x/2 = 4096 (no four-step)
+    with pytest.raises(topefft.TopeError) as e:
+        topefft.RealPlan(fw, (16384,), np.zeros(16384, np.float32), precision=topefft.PREC_F32)
+    assert e.value.code == topefft.TOPE_ERROR_UNSUPPORTED
+    fw.destroy()

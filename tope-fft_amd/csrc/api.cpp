@@ -784,9 +784,9 @@ int launch_pass(PlanImpl *P, size_t i, hipStream_t s, int dir)
         const void *src = P->buf[fwd ? ps.in_buf : ps.out_buf];
         void *dst = P->buf[fwd ? ps.out_buf : ps.in_buf];
         hipError_t err =
-            ps.real_stw ? launch_real_rows(fwd ? 0 : 1, src, dst, ps.real_stw, ps.real_w, P->real_m, P->real_pitch,
+            ps.real_stw ? launch_real_rows(P->prec, fwd ? 0 : 1, src, dst, ps.real_stw, ps.real_w, P->real_m, P->real_pitch,
                                            P->real_rows, P->real_scale, s)
-                        : launch_real_pass(fwd ? 0 : 1, src, dst, ps.real_w, P->real_m, P->real_pitch, P->real_rows,
+                        : launch_real_pass(P->prec, fwd ? 0 : 1, src, dst, ps.real_w, P->real_m, P->real_pitch, P->real_rows,
                                            P->real_scale, s);
         if (err != hipSuccess) return map_err(err);
     } else if (ps.kind == PASS_MFMA_TILE) {
@@ -1045,16 +1045,18 @@ void free_plan(PlanImpl *P)
 // in buf[1] over the columns 0..m (the Nyquist column m in a tile of its
 // own that runs first); INVERSE (C2R) runs the same passes in reverse order, the
 // pre-processing scaled by 1/(m y z) so the result carries 1/(x y z).
-int build_real_plan(struct topeFFT *f, long long nx, long long ny, long long nz, const double *real, PlanImpl **out,
-                    tope_ulong *pre_ns, tope_ulong *mem_ns)
+int build_real_plan(struct topeFFT *f, long long nx, long long ny, long long nz, int prec, const void *real,
+                    PlanImpl **out, tope_ulong *pre_ns, tope_ulong *mem_ns)
 {
     FrameworkImpl *fw = fw_of(f);
     if (!fw) return TOPE_ERROR_NOT_INITIALIZED;
     if (!real) return TOPE_ERROR_INVALID_ARG;
     if (nx < 2 || (nx & 1) || ny < 1 || nz < 1) return TOPE_ERROR_INVALID_SIZE;
-    // each axis (x/2 for the rows) a power of two up to 2^24, or 7-smooth up
-    // to 4096 (one mixed-radix pass)
-    auto covered = [](long long n) { return is_pow2(n) ? n <= kMaxAxis : mixed_supported(n); };
+    if (prec != PREC_F64 && prec != PREC_F32) return TOPE_ERROR_INVALID_ARG;
+    // each axis (x/2 for the rows) a power of two up to 2^24 (complex64: up
+    // to 4096, no four-step), or 7-smooth up to 4096 (one mixed-radix pass)
+    const long long pow2_max = prec == PREC_F64 ? kMaxAxis : (long long)kMaxFused;
+    auto covered = [&](long long n) { return is_pow2(n) ? n <= pow2_max : mixed_supported(n); };
     if (!covered(nx / 2) || !covered(ny) || !covered(nz)) return TOPE_ERROR_UNSUPPORTED;
     const long long m = nx / 2, rows = ny * nz;
     if ((m + 1) * rows > (1LL << 33)) return TOPE_ERROR_INVALID_SIZE;
@@ -1062,8 +1064,8 @@ int build_real_plan(struct topeFFT *f, long long nx, long long ny, long long nz,
     PlanImpl *P = new (std::nothrow) PlanImpl();
     if (!P) return TOPE_ERROR_UNSUPPORTED;
     P->fw = fw;
-    P->prec = PREC_F64;
-    P->elem = 16;
+    P->prec = prec;
+    P->elem = prec == PREC_F64 ? 16 : 8;
     P->ndim = nz > 1 ? 3 : (ny > 1 ? 2 : 1);
     P->dims[0] = nx;
     P->dims[1] = ny;
@@ -1077,7 +1079,7 @@ int build_real_plan(struct topeFFT *f, long long nx, long long ny, long long nz,
     P->real_rows = rows;
     P->real_pitch = pitch;
     P->real_scale = 1.0 / ((double)m * (double)rows);
-    P->real_bytes = (size_t)(nx * rows) * sizeof(double);
+    P->real_bytes = (size_t)(nx * rows) * (P->elem / 2);
     hipStream_t s = fw->stream;
 
     auto fail = [&](int rc) {
@@ -1741,7 +1743,8 @@ const char *topePlanPassName(void *plan_impl, int pass)
 
 /* ---- real-input transforms (no reference counterpart) ------------------- */
 
-void topeRealPlanInit(struct topeFFT *f, struct topePlanReal *t, int x, int y, int z, const double *real)
+void topeRealPlanInitEx(struct topeFFT *f, struct topePlanReal *t, int x, int y, int z, int precision,
+                        const void *real)
 {
     if (!f || !t) return;
     std::memset(t, 0, sizeof(*t));
@@ -1749,7 +1752,7 @@ void topeRealPlanInit(struct topeFFT *f, struct topePlanReal *t, int x, int y, i
     t->y = y;
     t->z = z;
     PlanImpl *P = nullptr;
-    const int rc = build_real_plan(f, x, y, z, real, &P, &t->totalPreKernel, &t->totalMemory);
+    const int rc = build_real_plan(f, x, y, z, precision, real, &P, &t->totalPreKernel, &t->totalMemory);
     f->error = rc;
     if (rc) return;
     P->fw->live_plans++;
@@ -1760,9 +1763,15 @@ void topeRealPlanInit(struct topeFFT *f, struct topePlanReal *t, int x, int y, i
     t->realSize = P->real_bytes;
     t->dataSize = P->bytes;
     t->pitch = (int)P->real_pitch;
+    t->precision = precision;
 }
 
-void topeRealExec(struct topeFFT *f, struct topePlanReal *t, double *host, int dir)
+void topeRealPlanInit(struct topeFFT *f, struct topePlanReal *t, int x, int y, int z, const double *real)
+{
+    topeRealPlanInitEx(f, t, x, y, z, PREC_F64, real);
+}
+
+void topeRealExecEx(struct topeFFT *f, struct topePlanReal *t, void *host, int dir)
 {
     if (!f || !t) return;
     PlanImpl *P = impl_of(t->impl);
@@ -1789,7 +1798,7 @@ void topeRealExec(struct topeFFT *f, struct topePlanReal *t, double *host, int d
         if (err == hipSuccess) {
             if (dir == FORWARD) {
                 // rows of m + 1 elements out of the padded device pitch
-                const size_t row = (size_t)(P->real_m + 1) * 16, dpitch = (size_t)P->real_pitch * 16;
+                const size_t row = (size_t)(P->real_m + 1) * P->elem, dpitch = (size_t)P->real_pitch * P->elem;
                 err = hipMemcpy2DAsync(host, row, P->buf[1], dpitch, row, (size_t)P->real_rows, hipMemcpyDeviceToHost,
                                        s);
             } else {
@@ -1802,6 +1811,11 @@ void topeRealExec(struct topeFFT *f, struct topePlanReal *t, double *host, int d
         rc = map_err(err);
     }
     f->error = rc;
+}
+
+void topeRealExec(struct topeFFT *f, struct topePlanReal *t, double *host, int dir)
+{
+    topeRealExecEx(f, t, host, dir);
 }
 
 void topeRealExecDevice(struct topeFFT *f, struct topePlanReal *t, int dir)

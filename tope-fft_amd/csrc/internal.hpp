@@ -68,14 +68,14 @@ hipError_t launch_mfma_rows(int mode, int n, const void *in, void *out, const Mf
 // post-processing (Z, row pitch m -> X, m + 1 values per row at row pitch
 // `pitch`), 1 = C2R pre-processing (X -> Z, scaled by s); w = W_n^k,
 // k = 0..m, n = 2m.
-hipError_t launch_real_pass(int dir, const void *in, void *out, const void *w, long long m, long long pitch,
+hipError_t launch_real_pass(int prec, int dir, const void *in, void *out, const void *w, long long m, long long pitch,
                             long long rows, double s, hipStream_t st);
 // The m-point row transform fused with that step (64 <= m <= 4096, powers of
 // two): dir 0 reads the real rows (pitch m complex) and writes the half
 // spectrum, dir 1 the reverse (scaled by s); stw is the stage twiddle table
 // of schedule (m, 16).
 bool real_rows_supported(long long m);
-hipError_t launch_real_rows(int dir, const void *in, void *out, const void *stw, const void *w, long long m,
+hipError_t launch_real_rows(int prec, int dir, const void *in, void *out, const void *stw, const void *w, long long m,
                             long long pitch, long long rows, double s, hipStream_t st);
 // Bluestein chirp-z passes for non-power-of-two axes (kernels_chirp.hip),
 // FP64: n the axis length, M the power-of-two convolution length; the lines

@@ -10,55 +10,58 @@
 // The inverse recovers E, O from X and its mirror,
 //   E[k] = (X[k] + conj X[m-k]) / 2,  O[k] = (X[k] - conj X[m-k]) W_n^-k / 2,
 // forms Z[k] = E[k] + i O[k] (k < m) and the inverse m-point complex pass
-// returns z, i.e. the even and odd real samples.  W_n^k (k <= m) comes from an
-// FP64 table with exactly reduced phases.
+// returns z, i.e. the even and odd real samples.  W_n^k (k <= m) comes from a
+// table with exactly reduced phases, generated in FP64 (rounded once for
+// complex64 storage).
 #include "internal.hpp"
 
 namespace tope {
 
 // One thread per mirror pair (k, m - k), k = 0..m/2, of one row: forward
 // reads Z (row pitch m) and writes X (m + 1 values per row, row pitch p).
-__global__ void r2c_post_kernel(const cpx<double> *__restrict__ Z, cpx<double> *__restrict__ X,
-                                const cpx<double> *__restrict__ w, long long m, long long p, long long rows)
+template <typename R>
+__global__ void r2c_post_kernel(const cpx<R> *__restrict__ Z, cpx<R> *__restrict__ X,
+                                const cpx<R> *__restrict__ w, long long m, long long p, long long rows)
 {
     const long long pairs = m / 2 + 1;
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= pairs * rows) return;
     const long long row = idx / pairs, k = idx % pairs;
-    const cpx<double> *z = Z + row * m;
-    cpx<double> *x = X + row * p;
-    const cpx<double> zk = z[k % m], zm = z[(m - k) % m];
+    const cpx<R> *z = Z + row * m;
+    cpx<R> *x = X + row * p;
+    const cpx<R> zk = z[k % m], zm = z[(m - k) % m];
     // X[k] from a = Z[k], b = conj Z[m-k]; X[m-k] from a' = Z[m-k], b' = conj Z[k]
-    auto out = [&](cpx<double> a, cpx<double> bc, long long kk) {
-        const cpx<double> b{bc.x, -bc.y};
-        const cpx<double> e{0.5 * (a.x + b.x), 0.5 * (a.y + b.y)};
+    auto out = [&](cpx<R> a, cpx<R> bc, long long kk) {
+        const cpx<R> b{bc.x, -bc.y};
+        const cpx<R> e{R(0.5) * (a.x + b.x), R(0.5) * (a.y + b.y)};
         // O = (a - b) / (2i) = (Im(a-b), -Re(a-b)) / 2
-        const cpx<double> o{0.5 * (a.y - b.y), -0.5 * (a.x - b.x)};
-        const cpx<double> wo = cmul(w[kk], o);
-        x[kk] = cpx<double>{e.x + wo.x, e.y + wo.y};
+        const cpx<R> o{R(0.5) * (a.y - b.y), R(-0.5) * (a.x - b.x)};
+        const cpx<R> wo = cmul(w[kk], o);
+        x[kk] = cpx<R>{e.x + wo.x, e.y + wo.y};
     };
     out(zk, zm, k);
     if (m - k != k) out(zm, zk, m - k);
 }
 
 // Inverse: X (row pitch p) -> Z (row pitch m), scaled by s; one thread per pair.
-__global__ void c2r_pre_kernel(const cpx<double> *__restrict__ X, cpx<double> *__restrict__ Z,
-                               const cpx<double> *__restrict__ w, long long m, long long p, long long rows, double s)
+template <typename R>
+__global__ void c2r_pre_kernel(const cpx<R> *__restrict__ X, cpx<R> *__restrict__ Z,
+                               const cpx<R> *__restrict__ w, long long m, long long p, long long rows, R s)
 {
     const long long pairs = m / 2 + 1;
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= pairs * rows) return;
     const long long row = idx / pairs, k = idx % pairs;
-    const cpx<double> *x = X + row * p;
-    cpx<double> *z = Z + row * m;
-    const cpx<double> xk = x[k], xm = x[m - k];
-    auto out = [&](cpx<double> a, cpx<double> bc, long long kk) {
-        const cpx<double> b{bc.x, -bc.y};
-        const cpx<double> e{0.5 * (a.x + b.x), 0.5 * (a.y + b.y)};
-        const cpx<double> d{0.5 * (a.x - b.x), 0.5 * (a.y - b.y)};
-        const cpx<double> o = cmulc(d, w[kk]);  // d * conj(W^k)
+    const cpx<R> *x = X + row * p;
+    cpx<R> *z = Z + row * m;
+    const cpx<R> xk = x[k], xm = x[m - k];
+    auto out = [&](cpx<R> a, cpx<R> bc, long long kk) {
+        const cpx<R> b{bc.x, -bc.y};
+        const cpx<R> e{R(0.5) * (a.x + b.x), R(0.5) * (a.y + b.y)};
+        const cpx<R> d{R(0.5) * (a.x - b.x), R(0.5) * (a.y - b.y)};
+        const cpx<R> o = cmulc(d, w[kk]);  // d * conj(W^k)
         // Z = E + i O
-        z[kk] = cpx<double>{s * (e.x - o.y), s * (e.y + o.x)};
+        z[kk] = cpx<R>{s * (e.x - o.y), s * (e.y + o.x)};
     };
     if (k < m) out(xk, xm, k);
     if (m - k != k && m - k < m) out(xm, xk, m - k);
@@ -73,53 +76,53 @@ __global__ void c2r_pre_kernel(const cpx<double> *__restrict__ X, cpx<double> *_
 //   0), store the row of m + 1 values at `pitch`.
 //   C2R (SIGN = +1): load X[k] and X[m - k] for its k, form
 //   Z[k] = s (E[k] + i O[k]), run the inverse radix stages, store z.
-template <int N, int E, int TB, int SIGN>
+template <typename R, int N, int E, int TB, int SIGN>
 __global__ void __launch_bounds__(TB *(N / E))
-real_rows_kernel(const cpx<double> *__restrict__ in, cpx<double> *__restrict__ out,
-                 const cpx<double> *__restrict__ stw, const cpx<double> *__restrict__ w, long long pitch,
-                 long long rows, double s)
+real_rows_kernel(const cpx<R> *__restrict__ in, cpx<R> *__restrict__ out,
+                 const cpx<R> *__restrict__ stw, const cpx<R> *__restrict__ w, long long pitch,
+                 long long rows, R s)
 {
     using SC = Sched<N, E>;
     constexpr int P = SC::P;
     constexpr long long m = N;
-    __shared__ __align__(16) unsigned char lds[Exchange<double, N, E, TB, false, true>::BYTES];
+    __shared__ __align__(16) unsigned char lds[Exchange<R, N, E, TB, false, true>::BYTES];
     const int tid = threadIdx.x;
     const int t = tid / P, p = tid % P;
     const long long row = (long long)blockIdx.x * TB + t;
     const bool valid = row < rows;
-    cpx<double> v[E];
+    cpx<R> v[E];
     if constexpr (SIGN < 0) {
-        const cpx<double> *z = in + row * m;
+        const cpx<R> *z = in + row * m;
 #pragma unroll
-        for (int e = 0; e < E; ++e) v[e] = valid ? z[p + P * e] : cpx<double>{0, 0};
+        for (int e = 0; e < E; ++e) v[e] = valid ? z[p + P * e] : cpx<R>{0, 0};
     } else {
-        const cpx<double> *X = in + row * pitch;
+        const cpx<R> *X = in + row * pitch;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const long long k = p + P * e;
-            cpx<double> xk{0, 0}, xm{0, 0};
+            cpx<R> xk{0, 0}, xm{0, 0};
             if (valid) {
                 xk = X[k];
                 xm = X[m - k];
             }
-            const cpx<double> b{xm.x, -xm.y};
-            const cpx<double> ev{0.5 * (xk.x + b.x), 0.5 * (xk.y + b.y)};
-            const cpx<double> d{0.5 * (xk.x - b.x), 0.5 * (xk.y - b.y)};
-            const cpx<double> o = cmulc(d, w[k]);
-            v[e] = cpx<double>{s * (ev.x - o.y), s * (ev.y + o.x)};
+            const cpx<R> b{xm.x, -xm.y};
+            const cpx<R> ev{R(0.5) * (xk.x + b.x), R(0.5) * (xk.y + b.y)};
+            const cpx<R> d{R(0.5) * (xk.x - b.x), R(0.5) * (xk.y - b.y)};
+            const cpx<R> o = cmulc(d, w[k]);
+            v[e] = cpx<R>{s * (ev.x - o.y), s * (ev.y + o.x)};
         }
     }
-    run_stages<double, N, E, TB, false, true, SIGN, 0>(v, p, t, stw, lds);
+    run_stages<R, N, E, TB, false, true, SIGN, 0>(v, p, t, stw, lds);
     if constexpr (SIGN > 0) {
         if (!valid) return;
-        cpx<double> *z = out + row * m;
+        cpx<R> *z = out + row * m;
 #pragma unroll
         for (int e = 0; e < E; ++e) z[p + P * e] = v[e];
     } else {
         // mirrors Z[(m - k) mod m] through LDS, one component at a time
-        double *l = reinterpret_cast<double *>(lds);
+        R *l = reinterpret_cast<R *>(lds);
         const int base = lds_index<N, E, TB, false>(t, 0);
-        cpx<double> mir[E];
+        cpx<R> mir[E];
 #pragma unroll
         for (int comp = 0; comp < 2; ++comp) {
             __syncthreads();
@@ -132,74 +135,91 @@ real_rows_kernel(const cpx<double> *__restrict__ in, cpx<double> *__restrict__ o
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const int k = (int)((m - (p + P * e)) & (m - 1));
-                const double val = l[base + k + (k >> 4)];
+                const R val = l[base + k + (k >> 4)];
                 if (comp) mir[e].y = val;
                 else mir[e].x = val;
             }
         }
         if (!valid) return;
-        cpx<double> *X = out + row * pitch;
+        cpx<R> *X = out + row * pitch;
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             const long long k = p + P * e;
-            const cpx<double> a = v[e], b{mir[e].x, -mir[e].y};
-            const cpx<double> ev{0.5 * (a.x + b.x), 0.5 * (a.y + b.y)};
-            const cpx<double> o{0.5 * (a.y - b.y), -0.5 * (a.x - b.x)};
-            const cpx<double> wo = cmul(w[k], o);
-            X[k] = cpx<double>{ev.x + wo.x, ev.y + wo.y};
+            const cpx<R> a = v[e], b{mir[e].x, -mir[e].y};
+            const cpx<R> ev{R(0.5) * (a.x + b.x), R(0.5) * (a.y + b.y)};
+            const cpx<R> o{R(0.5) * (a.y - b.y), R(-0.5) * (a.x - b.x)};
+            const cpx<R> wo = cmul(w[k], o);
+            X[k] = cpx<R>{ev.x + wo.x, ev.y + wo.y};
         }
-        if (p == 0) X[m] = cpx<double>{v[0].x - v[0].y, 0.0};  // k = m: E[0] - O[0] = Re Z0 - Im Z0
+        if (p == 0) X[m] = cpx<R>{v[0].x - v[0].y, R(0)};  // k = m: E[0] - O[0] = Re Z0 - Im Z0
     }
 }
 
-template <int N, int E, int TB>
+template <typename R, int N, int E, int TB>
 static hipError_t launch_rows_t(int dir, const void *in, void *out, const void *stw, const void *w,
                                 long long pitch, long long rows, double s, hipStream_t st)
 {
     const long long grid = (rows + TB - 1) / TB;
     if (grid > 0x7fffffffLL) return hipErrorInvalidValue;
-    const cpx<double> *i = static_cast<const cpx<double> *>(in);
-    cpx<double> *o = static_cast<cpx<double> *>(out);
-    const cpx<double> *t = static_cast<const cpx<double> *>(stw);
-    const cpx<double> *ww = static_cast<const cpx<double> *>(w);
+    const cpx<R> *i = static_cast<const cpx<R> *>(in);
+    cpx<R> *o = static_cast<cpx<R> *>(out);
+    const cpx<R> *t = static_cast<const cpx<R> *>(stw);
+    const cpx<R> *ww = static_cast<const cpx<R> *>(w);
     if (dir == 0)
-        hipLaunchKernelGGL((real_rows_kernel<N, E, TB, -1>), dim3((unsigned)grid), dim3(TB * (N / E)), 0, st, i, o, t,
-                           ww, pitch, rows, s);
+        hipLaunchKernelGGL((real_rows_kernel<R, N, E, TB, -1>), dim3((unsigned)grid), dim3(TB * (N / E)), 0, st, i, o, t,
+                           ww, pitch, rows, (R)s);
     else
-        hipLaunchKernelGGL((real_rows_kernel<N, E, TB, 1>), dim3((unsigned)grid), dim3(TB * (N / E)), 0, st, i, o, t,
-                           ww, pitch, rows, s);
+        hipLaunchKernelGGL((real_rows_kernel<R, N, E, TB, 1>), dim3((unsigned)grid), dim3(TB * (N / E)), 0, st, i, o, t,
+                           ww, pitch, rows, (R)s);
     return hipGetLastError();
 }
 
 bool real_rows_supported(long long m) { return m >= 64 && m <= 4096 && (m & (m - 1)) == 0; }
 
-hipError_t launch_real_rows(int dir, const void *in, void *out, const void *stw, const void *w, long long m,
-                            long long pitch, long long rows, double s, hipStream_t st)
+template <typename R>
+static hipError_t launch_real_rows_t(int dir, const void *in, void *out, const void *stw, const void *w, long long m,
+                                     long long pitch, long long rows, double s, hipStream_t st)
 {
     switch (m) {
-    case 64: return launch_rows_t<64, 16, 16>(dir, in, out, stw, w, pitch, rows, s, st);
-    case 128: return launch_rows_t<128, 16, 16>(dir, in, out, stw, w, pitch, rows, s, st);
-    case 256: return launch_rows_t<256, 16, 16>(dir, in, out, stw, w, pitch, rows, s, st);
-    case 512: return launch_rows_t<512, 16, 8>(dir, in, out, stw, w, pitch, rows, s, st);
-    case 1024: return launch_rows_t<1024, 16, 4>(dir, in, out, stw, w, pitch, rows, s, st);
-    case 2048: return launch_rows_t<2048, 16, 2>(dir, in, out, stw, w, pitch, rows, s, st);
-    case 4096: return launch_rows_t<4096, 16, 1>(dir, in, out, stw, w, pitch, rows, s, st);
+    case 64: return launch_rows_t<R, 64, 16, 16>(dir, in, out, stw, w, pitch, rows, s, st);
+    case 128: return launch_rows_t<R, 128, 16, 16>(dir, in, out, stw, w, pitch, rows, s, st);
+    case 256: return launch_rows_t<R, 256, 16, 16>(dir, in, out, stw, w, pitch, rows, s, st);
+    case 512: return launch_rows_t<R, 512, 16, 8>(dir, in, out, stw, w, pitch, rows, s, st);
+    case 1024: return launch_rows_t<R, 1024, 16, 4>(dir, in, out, stw, w, pitch, rows, s, st);
+    case 2048: return launch_rows_t<R, 2048, 16, 2>(dir, in, out, stw, w, pitch, rows, s, st);
+    case 4096: return launch_rows_t<R, 4096, 16, 1>(dir, in, out, stw, w, pitch, rows, s, st);
     default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_real_pass(int dir, const void *in, void *out, const void *w, long long m, long long pitch,
+hipError_t launch_real_rows(int prec, int dir, const void *in, void *out, const void *stw, const void *w, long long m,
+                            long long pitch, long long rows, double s, hipStream_t st)
+{
+    return prec == PREC_F64 ? launch_real_rows_t<double>(dir, in, out, stw, w, m, pitch, rows, s, st)
+                            : launch_real_rows_t<float>(dir, in, out, stw, w, m, pitch, rows, s, st);
+}
+
+template <typename R>
+static void launch_real_pass_t(int dir, const void *in, void *out, const void *w, long long m, long long pitch,
+                               long long rows, double s, long long grid, hipStream_t st)
+{
+    const cpx<R> *i = static_cast<const cpx<R> *>(in);
+    cpx<R> *o = static_cast<cpx<R> *>(out);
+    const cpx<R> *ww = static_cast<const cpx<R> *>(w);
+    if (dir == 0)
+        hipLaunchKernelGGL(r2c_post_kernel<R>, dim3((unsigned)grid), dim3(256), 0, st, i, o, ww, m, pitch, rows);
+    else
+        hipLaunchKernelGGL(c2r_pre_kernel<R>, dim3((unsigned)grid), dim3(256), 0, st, i, o, ww, m, pitch, rows, (R)s);
+}
+
+hipError_t launch_real_pass(int prec, int dir, const void *in, void *out, const void *w, long long m, long long pitch,
                             long long rows, double s, hipStream_t st)
 {
     const long long threads = (m / 2 + 1) * rows;
     const long long grid = (threads + 255) / 256;
     if (grid > 0x7fffffffLL) return hipErrorInvalidValue;
-    if (dir == 0)
-        hipLaunchKernelGGL(r2c_post_kernel, dim3((unsigned)grid), dim3(256), 0, st, static_cast<const cpx<double> *>(in),
-                           static_cast<cpx<double> *>(out), static_cast<const cpx<double> *>(w), m, pitch, rows);
-    else
-        hipLaunchKernelGGL(c2r_pre_kernel, dim3((unsigned)grid), dim3(256), 0, st, static_cast<const cpx<double> *>(in),
-                           static_cast<cpx<double> *>(out), static_cast<const cpx<double> *>(w), m, pitch, rows, s);
+    if (prec == PREC_F64) launch_real_pass_t<double>(dir, in, out, w, m, pitch, rows, s, grid, st);
+    else launch_real_pass_t<float>(dir, in, out, w, m, pitch, rows, s, grid, st);
     return hipGetLastError();
 }
 

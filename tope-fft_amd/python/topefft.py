@@ -69,7 +69,7 @@ class TopePlanMany(C.Structure):
 
 class TopePlanReal(C.Structure):
     _fields_ = [("x", C.c_int), ("y", C.c_int), ("z", C.c_int), ("dim", C.c_int),
-                ("pitch", C.c_int), ("reserved", C.c_int), ("data", C.c_void_p), ("real", C.c_void_p), ("dataSize", C.c_size_t), ("realSize", C.c_size_t),
+                ("pitch", C.c_int), ("precision", C.c_int), ("data", C.c_void_p), ("real", C.c_void_p), ("dataSize", C.c_size_t), ("realSize", C.c_size_t),
                 ("totalMemory", C.c_uint64), ("totalPreKernel", C.c_uint64), ("totalKernel", C.c_uint64),
                 ("impl", C.c_void_p)]
 
@@ -95,6 +95,8 @@ SIGNATURES = {
     "topeRealExec": (None, [_F, _PR, _DP, C.c_int]),
     "topeRealExecDevice": (None, [_F, _PR, C.c_int]),
     "topeRealExecPtr": (None, [_F, _PR, C.c_void_p, C.c_void_p, C.c_int]),
+    "topeRealPlanInitEx": (None, [_F, _PR, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]),
+    "topeRealExecEx": (None, [_F, _PR, C.c_void_p, C.c_int]),
     "topeRealDestroy": (None, [_F, _PR]),
     "topeFFTInit": (None, [_F]),
     "topeFFTDestroy": (None, [_F]),
@@ -393,30 +395,40 @@ class RealPlan:
     (nx[, ny[, nz]]), x fastest; forward() gives the half spectrum of
     numpy's rfftn layout (nz, ny, nx//2 + 1), inverse() the real array."""
 
-    def __init__(self, fw, dims, x):
+    def __init__(self, fw, dims, x, precision=None):
         self.fw, self.lib = fw, fw.lib
         d = tuple(int(v) for v in dims) + (1,) * (3 - len(dims))
         self.dims = d
         self.n = d[0] * d[1] * d[2]
         self.half = (d[0] // 2 + 1) * d[1] * d[2]
-        host = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        self.precision = precision
+        self.rdtype = np.float32 if precision == PREC_F32 else np.float64
+        self.cdtype = np.complex64 if precision == PREC_F32 else np.complex128
+        host = np.ascontiguousarray(x, dtype=self.rdtype).reshape(-1)
         if host.size != self.n:
             raise ValueError("input size does not match dims")
         self.t = TopePlanReal()
-        self.lib.topeRealPlanInit(C.byref(fw.f), C.byref(self.t), d[0], d[1], d[2], _dptr(host))
+        if precision is None:
+            self.lib.topeRealPlanInit(C.byref(fw.f), C.byref(self.t), d[0], d[1], d[2], _dptr(host))
+        else:
+            self.lib.topeRealPlanInitEx(C.byref(fw.f), C.byref(self.t), d[0], d[1], d[2], precision,
+                                        host.ctypes.data_as(C.c_void_p))
         fw.check(f"topeRealPlanInit{d}")
 
-    def forward(self):
-        out = np.empty(self.half, np.complex128)
-        self.lib.topeRealExec(C.byref(self.fw.f), C.byref(self.t), _dptr(out.view(np.float64)), FORWARD)
-        self.fw.check("topeRealExec(FORWARD)")
+    def _exec(self, out, direction):
+        if self.precision is None:
+            self.lib.topeRealExec(C.byref(self.fw.f), C.byref(self.t), _dptr(out.view(np.float64)), direction)
+        else:
+            self.lib.topeRealExecEx(C.byref(self.fw.f), C.byref(self.t), out.ctypes.data_as(C.c_void_p),
+                                    direction)
+        self.fw.check("topeRealExec(%s)" % ("FORWARD" if direction == FORWARD else "INVERSE"))
         return out
 
+    def forward(self):
+        return self._exec(np.empty(self.half, self.cdtype), FORWARD)
+
     def inverse(self):
-        out = np.empty(self.n, np.float64)
-        self.lib.topeRealExec(C.byref(self.fw.f), C.byref(self.t), _dptr(out), INVERSE)
-        self.fw.check("topeRealExec(INVERSE)")
-        return out
+        return self._exec(np.empty(self.n, self.rdtype), INVERSE)
 
     def exec_device(self, direction=FORWARD):
         self.lib.topeRealExecDevice(C.byref(self.fw.f), C.byref(self.t), direction)
