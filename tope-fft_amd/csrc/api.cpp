@@ -508,7 +508,7 @@ class Planner {
         ps.mx.roots = roots;
         ps.grid = mixed_grid(ps.mx);
         ps.in_buf = ps.out_buf = cur;
-        ps.name = "mixed_radix_f64";
+        ps.name = P->prec == PREC_F64 ? "mixed_radix_f64" : "mixed_radix_f32";
         P->passes.push_back(ps);
         if (axis_id >= 0 && !P->axis_tw[axis_id].table) {
             P->axis_tw[axis_id].n = (int)n;
