@@ -299,7 +299,7 @@ void topeRealExec(struct topeFFT *f, struct topePlanReal *t, double *host, int d
 void topeRealExecDevice(struct topeFFT *f, struct topePlanReal *t, int dir);
 /* Precision variant: TOPE_PREC_F64 (as above) or TOPE_PREC_F32 (float real
  * array, complex64 half spectrum, FP32 arithmetic with FP64-generated
- * tables rounded once; x/2, y, z powers of two or 7-smooth up to 4096).
+ * tables rounded once; the same sizes).
  * ExecEx downloads into `host` as Exec does, in the plan's precision. */
 void topeRealPlanInitEx(struct topeFFT *f, struct topePlanReal *t, int x, int y, int z, int precision,
                         const void *real);

@@ -121,9 +121,10 @@ def test_batch_round_trip_mfma():
     assert rel_l2(back.astype(np.complex128), x.astype(np.complex128)) < 2e-6
 
 
-@pytest.mark.parametrize("n", [12, 100, 1000, 3000])
+@pytest.mark.parametrize("n", [12, 100, 1000, 3000, 8192, 16384])
 def test_batch_complex64_mixed_radix(oracle, n):
-    # complex64 batch of 7-smooth lengths: the mixed-radix pass in FP32
+    # complex64 batch of 7-smooth lengths (the mixed-radix pass in FP32) and
+    # of powers of two above 4096 (the four-step in FP32)
     count = 10
     x = batch_input(n, count, 300 + n).astype(np.complex64)
     x64 = x.astype(np.complex128)

@@ -370,7 +370,8 @@ def test_2d_dif2_schedule_small(oracle, monkeypatch, dims):
 
 
 @pytest.mark.parametrize("dims", [(16, 16), (64, 32), (8, 16, 4), (32, 32, 32), (64, 4, 8),
-                                  (12, 10), (100, 6), (15, 7, 9), (1000, 3), (3000, 2)],
+                                  (12, 10), (100, 6), (15, 7, 9), (1000, 3), (3000, 2),
+                                  (8192, 4), (4, 8192), (16384, 2), (2, 2, 8192)],
                          ids=lambda d: "x".join(map(str, d)))
 def test_nd_complex64(oracle, dims):
     # complex64 storage plans (tope2DPlanInitEx / tope3DPlanInitEx); 7-smooth

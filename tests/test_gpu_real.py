@@ -129,7 +129,8 @@ def test_r2c_large_four_step_bins(oracle):
     assert rel_l2(back, x) < 1e-12
 
 
-C64_SHAPES = [(256,), (4096,), (8192,), (64, 32), (256, 256), (16, 8, 4), (2, 8), (200,), (24, 10), (30, 6, 10)]
+C64_SHAPES = [(256,), (4096,), (8192,), (64, 32), (256, 256), (16, 8, 4), (2, 8), (200,), (24, 10), (30, 6, 10),
+              (16384,), (32768,), (16, 8192)]
 
 
 @pytest.mark.parametrize("dims", C64_SHAPES, ids=lambda d: "x".join(map(str, d)))
@@ -155,8 +156,10 @@ def test_r2c_c2r_complex64(oracle, dims):
 
 def test_real_complex64_limits():
     fw = topefft.Framework()
-    # complex64 rows stop at x/2 = 4096 (no four-step)
     with pytest.raises(topefft.TopeError) as e:
-        topefft.RealPlan(fw, (16384,), np.zeros(16384, np.float32), precision=topefft.PREC_F32)
+        topefft.RealPlan(fw, (22,), np.zeros(22, np.float32), precision=topefft.PREC_F32)
     assert e.value.code == topefft.TOPE_ERROR_UNSUPPORTED
+    with pytest.raises(topefft.TopeError) as e:
+        topefft.RealPlan(fw, (16,), np.zeros(16, np.float32), precision=5)
+    assert e.value.code == topefft.TOPE_ERROR_INVALID_ARG
     fw.destroy()

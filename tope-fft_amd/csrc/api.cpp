@@ -1053,10 +1053,9 @@ int build_real_plan(struct topeFFT *f, long long nx, long long ny, long long nz,
     if (!real) return TOPE_ERROR_INVALID_ARG;
     if (nx < 2 || (nx & 1) || ny < 1 || nz < 1) return TOPE_ERROR_INVALID_SIZE;
     if (prec != PREC_F64 && prec != PREC_F32) return TOPE_ERROR_INVALID_ARG;
-    // each axis (x/2 for the rows) a power of two up to 2^24 (complex64: up
-    // to 4096, no four-step), or 7-smooth up to 4096 (one mixed-radix pass)
-    const long long pow2_max = prec == PREC_F64 ? kMaxAxis : (long long)kMaxFused;
-    auto covered = [&](long long n) { return is_pow2(n) ? n <= pow2_max : mixed_supported(n); };
+    // each axis (x/2 for the rows) a power of two up to 2^24, or 7-smooth up
+    // to 4096 (one mixed-radix pass)
+    auto covered = [](long long n) { return is_pow2(n) ? n <= kMaxAxis : mixed_supported(n); };
     if (!covered(nx / 2) || !covered(ny) || !covered(nz)) return TOPE_ERROR_UNSUPPORTED;
     const long long m = nx / 2, rows = ny * nz;
     if ((m + 1) * rows > (1LL << 33)) return TOPE_ERROR_INVALID_SIZE;
