@@ -305,9 +305,9 @@ void topeRealPlanInitEx(struct topeFFT *f, struct topePlanReal *t, int x, int y,
                         const void *real);
 void topeRealExecEx(struct topeFFT *f, struct topePlanReal *t, void *host, int dir);
 /* topeRealExecPtr: the same transform on caller-owned device arrays, in
- * the plan's device layout: `real` holds x*y*z doubles (realSize bytes),
- * `spectrum` dataSize bytes (rows of x/2+1 complex128 at a pitch of `pitch`
- * elements).  Asynchronous on the framework's stream, no host copies.
+ * the plan's device layout and precision: `real` holds x*y*z values
+ * (realSize bytes), `spectrum` dataSize bytes (rows of x/2+1 complex values
+ * at a pitch of `pitch` elements).  Asynchronous on the framework's stream, no host copies.
  * FORWARD writes `spectrum` and may overwrite `real`; INVERSE writes `real`
  * and may overwrite `spectrum`.  Plans with y or z above 4096 use the real
  * array's buffer as dataSize bytes of scratch and are refused
