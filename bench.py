@@ -210,9 +210,23 @@ def real_2d_leg(fw, steps, warmup):
     names = p.pass_names()
     p.destroy()
     alg = 2 * 2 * 16 * (nx // 2 + 1) * ny
-    return {"workload": "2D R2C/C2R FP64 4096x4096 real, device-resident", "ms": round(t * 1e3, 4),
-            "transforms_per_s": round(1.0 / t, 2), "gbs_algorithmic": round(gbs(alg, t), 1),
-            "passes": [{"kernel": nm, "ms": round(float(m), 4)} for nm, m in zip(names, pass_ms)]}
+    out = {"workload": "2D R2C/C2R FP64 4096x4096 real, device-resident", "ms": round(t * 1e3, 4),
+           "transforms_per_s": round(1.0 / t, 2), "gbs_algorithmic": round(gbs(alg, t), 1),
+           "passes": [{"kernel": nm, "ms": round(float(m), 4)} for nm, m in zip(names, pass_ms)]}
+    # complex64 storage on the same input: time and error against the FP64
+    # half spectrum of one forward transform
+    p64 = topefft.RealPlan(fw, (nx, ny), x)
+    X64 = p64.forward()
+    p64.destroy()
+    q = topefft.RealPlan(fw, (nx, ny), x.astype(np.float32), precision=topefft.PREC_F32)
+    X32 = q.forward().astype(np.complex128)
+    t32, pass32 = measure(fw, q, steps, warmup)
+    q.destroy()
+    err = float(np.linalg.norm(X32 - X64) / np.linalg.norm(X64))
+    out["complex64"] = {"ms": round(t32 * 1e3, 4), "transforms_per_s": round(1.0 / t32, 2),
+                        "gbs_algorithmic": round(gbs(alg // 2, t32), 1), "speedup_vs_fp64": round(t / t32, 3),
+                        "rel_l2_vs_fp64": float("%.3e" % err)}
+    return out
 
 
 def small_1d_leg(fw, steps, warmup):
