@@ -64,7 +64,7 @@ __device__ __forceinline__ long long divmod64(long long a, const Div64 &d, long 
 }
 
 template <int SIGN, typename R>
-__device__ __forceinline__ cpx<R> root(const cpx<R> *__restrict__ w, long long i)
+__device__ __forceinline__ cpx<R> root(const cpx<R> *__restrict__ w, int i)
 {
     const cpx<R> v = w[i];
     return SIGN < 0 ? v : cpx<R>{v.x, -v.y};
@@ -74,7 +74,7 @@ __device__ __forceinline__ cpx<R> root(const cpx<R> *__restrict__ w, long long i
 // b_j = x_j - x_{r-j} and W_r^{jk} = c + i s,
 //   y_k = x_0 + sum_{j=1}^{(r-1)/2} (a_j c + i b_j s).
 template <int r, int SIGN, typename R>
-__device__ __forceinline__ void dft_odd(cpx<R> (&x)[r], const cpx<R> *__restrict__ w, long long step)
+__device__ __forceinline__ void dft_odd(cpx<R> (&x)[r], const cpx<R> *__restrict__ w, int step)
 {
     constexpr int h = (r - 1) / 2;
     cpx<R> a[h], b[h], wr[r];
@@ -114,7 +114,7 @@ __device__ __forceinline__ void mixed_stage(const cpx<R> *src, cpx<R> *dst, int 
                                             const cpx<R> *__restrict__ roots)
 {
     const int nb = n / r;  // butterflies per line
-    const long long tws = (long long)n / ((long long)Ns * r);
+    const int tws = n / (Ns * r);  // table indices stay below n <= 4096: 32-bit
     const Div32 dnb = mkdiv(nb), dns = mkdiv(Ns);
     for (int bi = threadIdx.x; bi < tb * nb; bi += NT) {
         int j, kk;
@@ -126,7 +126,7 @@ __device__ __forceinline__ void mixed_stage(const cpx<R> *src, cpx<R> *dst, int 
         for (int m = 0; m < r; ++m) x[m] = src[lslot(si + m * nb)];
         if (Ns > 1) {
 #pragma unroll
-            for (int m = 1; m < r; ++m) x[m] = cmul(x[m], root<SIGN>(roots, (long long)kk * m * tws));
+            for (int m = 1; m < r; ++m) x[m] = cmul(x[m], root<SIGN>(roots, kk * m * tws));
         }
         if constexpr (r == 3 || r == 5 || r == 7) dft_odd<r, SIGN>(x, roots, n / r);
         else dft<r, SIGN>(x);
