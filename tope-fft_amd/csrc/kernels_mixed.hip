@@ -11,9 +11,10 @@
 // r-point DFT to (j / Ns) Ns r + (j mod Ns) + m Ns; starting at Ns = 1 the
 // result is in natural order.  A block owns whole lines: it loads them into
 // LDS, runs every stage between two LDS buffers (a barrier per stage), and
-// writes them back, so the pass is in place.  Every twiddle and every
-// radix-3/5/7 constant is an entry of the plan's table W_n^i (exactly
-// reduced phases, forward sign), so no device sin/cos is evaluated.
+// writes them back, so the pass is in place.  Every twiddle is an entry of
+// the plan's table W_n^i (exactly reduced phases, forward sign) and the
+// radix-3/5/7 constants are correctly rounded literals, so no device sin/cos
+// is evaluated.
 #include <algorithm>
 
 #include "internal.hpp"
@@ -70,11 +71,45 @@ __device__ __forceinline__ cpx<R> root(const cpx<R> *__restrict__ w, int i)
     return SIGN < 0 ? v : cpx<R>{v.x, -v.y};
 }
 
+// cos(2 pi m / r) and sin(2 pi m / r), m < r, for the odd radices: FP64
+// values correctly rounded from 50-digit evaluations (complex64 rounds them
+// once more).
+template <int r>
+__device__ __forceinline__ double odd_cos(int m)
+{
+    constexpr double c3[3] = {0x1p+0, -0x1p-1, -0x1p-1};
+    constexpr double c5[5] = {0x1p+0, 0x1.3c6ef372fe950p-2, -0x1.9e3779b97f4a8p-1, -0x1.9e3779b97f4a8p-1,
+                              0x1.3c6ef372fe950p-2};
+    constexpr double c7[7] = {0x1p+0,
+                              0x1.3f3a0e28bedd1p-1,
+                              -0x1.c7b90e3024582p-3,
+                              -0x1.cd4bca9cb5c71p-1,
+                              -0x1.cd4bca9cb5c71p-1,
+                              -0x1.c7b90e3024582p-3,
+                              0x1.3f3a0e28bedd1p-1};
+    return r == 3 ? c3[m] : r == 5 ? c5[m] : c7[m];
+}
+template <int r>
+__device__ __forceinline__ double odd_sin(int m)
+{
+    constexpr double s3[3] = {0.0, 0x1.bb67ae8584caap-1, -0x1.bb67ae8584caap-1};
+    constexpr double s5[5] = {0.0, 0x1.e6f0e134454ffp-1, 0x1.2cf2304755a5ep-1, -0x1.2cf2304755a5ep-1,
+                              -0x1.e6f0e134454ffp-1};
+    constexpr double s7[7] = {0.0,
+                              0x1.904c37505de4bp-1,
+                              0x1.f329c0558e969p-1,
+                              0x1.bc4c04d71abc1p-2,
+                              -0x1.bc4c04d71abc1p-2,
+                              -0x1.f329c0558e969p-1,
+                              -0x1.904c37505de4bp-1};
+    return r == 3 ? s3[m] : r == 5 ? s5[m] : s7[m];
+}
+
 // Odd prime radix r by the conjugate-pair form: with a_j = x_j + x_{r-j},
 // b_j = x_j - x_{r-j} and W_r^{jk} = c + i s,
 //   y_k = x_0 + sum_{j=1}^{(r-1)/2} (a_j c + i b_j s).
 template <int r, int SIGN, typename R>
-__device__ __forceinline__ void dft_odd(cpx<R> (&x)[r], const cpx<R> *__restrict__ w, int step)
+__device__ __forceinline__ void dft_odd(cpx<R> (&x)[r])
 {
     constexpr int h = (r - 1) / 2;
     cpx<R> a[h], b[h], wr[r];
@@ -83,8 +118,10 @@ __device__ __forceinline__ void dft_odd(cpx<R> (&x)[r], const cpx<R> *__restrict
         a[j - 1] = x[j] + x[r - j];
         b[j - 1] = x[j] - x[r - j];
     }
+    // W_r^m = cos - i sin forward, its conjugate inverse
 #pragma unroll
-    for (int m = 1; m < r; ++m) wr[m] = root<SIGN>(w, m * step);
+    for (int m = 1; m < r; ++m)
+        wr[m] = cpx<R>{(R)odd_cos<r>(m), (R)(SIGN < 0 ? -odd_sin<r>(m) : odd_sin<r>(m))};
     cpx<R> y[r];
     y[0] = x[0];
 #pragma unroll
@@ -128,7 +165,7 @@ __device__ __forceinline__ void mixed_stage(const cpx<R> *src, cpx<R> *dst, int 
 #pragma unroll
             for (int m = 1; m < r; ++m) x[m] = cmul(x[m], root<SIGN>(roots, kk * m * tws));
         }
-        if constexpr (r == 3 || r == 5 || r == 7) dft_odd<r, SIGN>(x, roots, n / r);
+        if constexpr (r == 3 || r == 5 || r == 7) dft_odd<r, SIGN>(x);
         else dft<r, SIGN>(x);
         const int di = t * n + jn * Ns * r + kk;
 #pragma unroll
