@@ -145,10 +145,21 @@ __device__ __forceinline__ void dft_odd(cpx<R> (&x)[r])
 // the power-of-two strides of the stage scatter (r, Ns) across the banks.
 __device__ __forceinline__ int lslot(int i) { return i + (i >> 4); }
 
-// One stage of compile-time radix r over the block's lines, LDS to LDS.
-template <typename R, int r, int SIGN, int NT>
+// Where the last stage of a contiguous (row) axis writes: straight to the
+// output lines, scaled (consecutive lanes hold consecutive outputs).
+template <typename R>
+struct RowOut {
+    cpx<R> *out;
+    long long l0, lines, os, cs;
+    Div64 dC;
+    R scale;
+};
+
+// One stage of compile-time radix r over the block's lines, LDS to LDS, or
+// (TOGLOBAL) LDS to the output rows.
+template <typename R, int r, int SIGN, int NT, bool TOGLOBAL>
 __device__ __forceinline__ void mixed_stage(const cpx<R> *src, cpx<R> *dst, int n, int tb, int Ns,
-                                            const cpx<R> *__restrict__ roots)
+                                            const cpx<R> *__restrict__ roots, const RowOut<R> &go)
 {
     const int nb = n / r;  // butterflies per line
     const int tws = n / (Ns * r);  // table indices stay below n <= 4096: 32-bit
@@ -167,9 +178,36 @@ __device__ __forceinline__ void mixed_stage(const cpx<R> *src, cpx<R> *dst, int 
         }
         if constexpr (r == 3 || r == 5 || r == 7) dft_odd<r, SIGN>(x);
         else dft<r, SIGN>(x);
-        const int di = t * n + jn * Ns * r + kk;
+        if constexpr (TOGLOBAL) {
+            const long long l = go.l0 + t;
+            if (l < go.lines) {
+                long long c;
+                const long long o = divmod64(l, go.dC, c);
+                cpx<R> *row = go.out + o * go.os + c * go.cs;
+                const int k0 = jn * Ns * r + kk;
 #pragma unroll
-        for (int m = 0; m < r; ++m) dst[lslot(di + m * Ns)] = x[m];
+                for (int m = 0; m < r; ++m) row[k0 + m * Ns] = cpx<R>{x[m].x * go.scale, x[m].y * go.scale};
+            }
+        } else {
+            const int di = t * n + jn * Ns * r + kk;
+#pragma unroll
+            for (int m = 0; m < r; ++m) dst[lslot(di + m * Ns)] = x[m];
+        }
+    }
+}
+
+template <typename R, int SIGN, int NT, bool TOGLOBAL>
+__device__ __forceinline__ void mixed_stage_r(int r, const cpx<R> *src, cpx<R> *dst, int n, int tb, int Ns,
+                                              const cpx<R> *__restrict__ roots, const RowOut<R> &go)
+{
+    switch (r) {
+    case 2: mixed_stage<R, 2, SIGN, NT, TOGLOBAL>(src, dst, n, tb, Ns, roots, go); break;
+    case 3: mixed_stage<R, 3, SIGN, NT, TOGLOBAL>(src, dst, n, tb, Ns, roots, go); break;
+    case 4: mixed_stage<R, 4, SIGN, NT, TOGLOBAL>(src, dst, n, tb, Ns, roots, go); break;
+    case 5: mixed_stage<R, 5, SIGN, NT, TOGLOBAL>(src, dst, n, tb, Ns, roots, go); break;
+    case 7: mixed_stage<R, 7, SIGN, NT, TOGLOBAL>(src, dst, n, tb, Ns, roots, go); break;
+    case 8: mixed_stage<R, 8, SIGN, NT, TOGLOBAL>(src, dst, n, tb, Ns, roots, go); break;
+    default: mixed_stage<R, 16, SIGN, NT, TOGLOBAL>(src, dst, n, tb, Ns, roots, go); break;
     }
 }
 
@@ -213,18 +251,18 @@ __global__ void __launch_bounds__(NT) mixed_kernel(const cpx<R> *in, cpx<R> *out
     __syncthreads();
 
     cpx<R> *src = lds, *dst = lds + SLOTS;
+    // rows: the last stage stores straight to the output (no final LDS
+    // round trip); strided axes keep the LDS pass so that neighbouring
+    // lanes store neighbouring lines
+    const RowOut<R> go{out, l0, a.lines, a.os, a.cs, dC, (R)a.scale};
     int Ns = 1;
     for (int s = 0; s < a.nst; ++s) {
         const int r = a.radix[s];
-        switch (r) {
-        case 2: mixed_stage<R, 2, SIGN, NT>(src, dst, n, tb, Ns, roots); break;
-        case 3: mixed_stage<R, 3, SIGN, NT>(src, dst, n, tb, Ns, roots); break;
-        case 4: mixed_stage<R, 4, SIGN, NT>(src, dst, n, tb, Ns, roots); break;
-        case 5: mixed_stage<R, 5, SIGN, NT>(src, dst, n, tb, Ns, roots); break;
-        case 7: mixed_stage<R, 7, SIGN, NT>(src, dst, n, tb, Ns, roots); break;
-        case 8: mixed_stage<R, 8, SIGN, NT>(src, dst, n, tb, Ns, roots); break;
-        default: mixed_stage<R, 16, SIGN, NT>(src, dst, n, tb, Ns, roots); break;
+        if (!cfast && s == a.nst - 1) {
+            mixed_stage_r<R, SIGN, NT, true>(r, src, dst, n, tb, Ns, roots, go);
+            return;
         }
+        mixed_stage_r<R, SIGN, NT, false>(r, src, dst, n, tb, Ns, roots, go);
         __syncthreads();
         cpx<R> *tmp = src;
         src = dst;
