@@ -145,19 +145,21 @@ __device__ __forceinline__ void dft_odd(cpx<R> (&x)[r])
 // the power-of-two strides of the stage scatter (r, Ns) across the banks.
 __device__ __forceinline__ int lslot(int i) { return i + (i >> 4); }
 
-// Where the last stage of a contiguous (row) axis writes: straight to the
-// output lines, scaled (consecutive lanes hold consecutive outputs).
+// Rows of a contiguous axis: where the first stage reads and the last
+// stage writes, straight from the input lines and to the output lines,
+// scaled (consecutive lanes hold consecutive elements).
 template <typename R>
 struct RowOut {
+    const cpx<R> *in;
     cpx<R> *out;
     long long l0, lines, os, cs;
     Div64 dC;
     R scale;
 };
 
-// One stage of compile-time radix r over the block's lines, LDS to LDS, or
-// (TOGLOBAL) LDS to the output rows.
-template <typename R, int r, int SIGN, int NT, bool TOGLOBAL>
+// One stage of compile-time radix r over the block's lines, LDS to LDS,
+// (FROMGLOBAL) the input rows to LDS, or (TOGLOBAL) LDS to the output rows.
+template <typename R, int r, int SIGN, int NT, bool TOGLOBAL, bool FROMGLOBAL = false>
 __device__ __forceinline__ void mixed_stage(const cpx<R> *src, cpx<R> *dst, int n, int tb, int Ns,
                                             const cpx<R> *__restrict__ roots, const RowOut<R> &go)
 {
@@ -169,9 +171,23 @@ __device__ __forceinline__ void mixed_stage(const cpx<R> *src, cpx<R> *dst, int 
         const int t = divmod(bi, dnb, j);
         const int jn = divmod(j, dns, kk);  // j = jn Ns + kk
         cpx<R> x[r];
-        const int si = t * n + j;
+        if constexpr (FROMGLOBAL) {
+            const long long l = go.l0 + t;
+            if (l < go.lines) {
+                long long c;
+                const long long o = divmod64(l, go.dC, c);
+                const cpx<R> *row = go.in + o * go.os + c * go.cs;
 #pragma unroll
-        for (int m = 0; m < r; ++m) x[m] = src[lslot(si + m * nb)];
+                for (int m = 0; m < r; ++m) x[m] = row[j + m * nb];
+            } else {
+#pragma unroll
+                for (int m = 0; m < r; ++m) x[m] = cpx<R>{0, 0};
+            }
+        } else {
+            const int si = t * n + j;
+#pragma unroll
+            for (int m = 0; m < r; ++m) x[m] = src[lslot(si + m * nb)];
+        }
         if (Ns > 1) {
 #pragma unroll
             for (int m = 1; m < r; ++m) x[m] = cmul(x[m], root<SIGN>(roots, kk * m * tws));
@@ -196,18 +212,18 @@ __device__ __forceinline__ void mixed_stage(const cpx<R> *src, cpx<R> *dst, int 
     }
 }
 
-template <typename R, int SIGN, int NT, bool TOGLOBAL>
+template <typename R, int SIGN, int NT, bool TOGLOBAL, bool FROMGLOBAL = false>
 __device__ __forceinline__ void mixed_stage_r(int r, const cpx<R> *src, cpx<R> *dst, int n, int tb, int Ns,
                                               const cpx<R> *__restrict__ roots, const RowOut<R> &go)
 {
     switch (r) {
-    case 2: mixed_stage<R, 2, SIGN, NT, TOGLOBAL>(src, dst, n, tb, Ns, roots, go); break;
-    case 3: mixed_stage<R, 3, SIGN, NT, TOGLOBAL>(src, dst, n, tb, Ns, roots, go); break;
-    case 4: mixed_stage<R, 4, SIGN, NT, TOGLOBAL>(src, dst, n, tb, Ns, roots, go); break;
-    case 5: mixed_stage<R, 5, SIGN, NT, TOGLOBAL>(src, dst, n, tb, Ns, roots, go); break;
-    case 7: mixed_stage<R, 7, SIGN, NT, TOGLOBAL>(src, dst, n, tb, Ns, roots, go); break;
-    case 8: mixed_stage<R, 8, SIGN, NT, TOGLOBAL>(src, dst, n, tb, Ns, roots, go); break;
-    default: mixed_stage<R, 16, SIGN, NT, TOGLOBAL>(src, dst, n, tb, Ns, roots, go); break;
+    case 2: mixed_stage<R, 2, SIGN, NT, TOGLOBAL, FROMGLOBAL>(src, dst, n, tb, Ns, roots, go); break;
+    case 3: mixed_stage<R, 3, SIGN, NT, TOGLOBAL, FROMGLOBAL>(src, dst, n, tb, Ns, roots, go); break;
+    case 4: mixed_stage<R, 4, SIGN, NT, TOGLOBAL, FROMGLOBAL>(src, dst, n, tb, Ns, roots, go); break;
+    case 5: mixed_stage<R, 5, SIGN, NT, TOGLOBAL, FROMGLOBAL>(src, dst, n, tb, Ns, roots, go); break;
+    case 7: mixed_stage<R, 7, SIGN, NT, TOGLOBAL, FROMGLOBAL>(src, dst, n, tb, Ns, roots, go); break;
+    case 8: mixed_stage<R, 8, SIGN, NT, TOGLOBAL, FROMGLOBAL>(src, dst, n, tb, Ns, roots, go); break;
+    default: mixed_stage<R, 16, SIGN, NT, TOGLOBAL, FROMGLOBAL>(src, dst, n, tb, Ns, roots, go); break;
     }
 }
 
@@ -242,21 +258,33 @@ __global__ void __launch_bounds__(NT) mixed_kernel(const cpx<R> *in, cpx<R> *out
         return o * a.os + c * a.cs + (long long)k * a.is;
     };
 
-    for (int idx = threadIdx.x; idx < span_elems; idx += NT) {
-        int t, k;
-        split(idx, t, k);
-        const long long l = l0 + t;
-        lds[lslot(t * n + k)] = l < a.lines ? in[gaddr(l, k)] : cpx<R>{0, 0};
-    }
-    __syncthreads();
-
+    // rows of a pass with two or more stages: the first stage reads the
+    // input lines itself (no initial LDS fill and barrier) and the last
+    // stores straight to the output (no final LDS round trip); strided axes
+    // keep both LDS passes so that neighbouring lanes touch neighbouring
+    // lines
+    const RowOut<R> go{in, out, l0, a.lines, a.os, a.cs, dC, (R)a.scale};
+    const bool rows_direct = !cfast && a.nst >= 2;
     cpx<R> *src = lds, *dst = lds + SLOTS;
-    // rows: the last stage stores straight to the output (no final LDS
-    // round trip); strided axes keep the LDS pass so that neighbouring
-    // lanes store neighbouring lines
-    const RowOut<R> go{out, l0, a.lines, a.os, a.cs, dC, (R)a.scale};
     int Ns = 1;
-    for (int s = 0; s < a.nst; ++s) {
+    int s = 0;
+    if (rows_direct) {
+        mixed_stage_r<R, SIGN, NT, false, true>(a.radix[0], src, dst, n, tb, Ns, roots, go);
+        __syncthreads();
+        src = lds + SLOTS;
+        dst = lds;
+        Ns = a.radix[0];
+        s = 1;
+    } else {
+        for (int idx = threadIdx.x; idx < span_elems; idx += NT) {
+            int t, k;
+            split(idx, t, k);
+            const long long l = l0 + t;
+            lds[lslot(t * n + k)] = l < a.lines ? in[gaddr(l, k)] : cpx<R>{0, 0};
+        }
+        __syncthreads();
+    }
+    for (; s < a.nst; ++s) {
         const int r = a.radix[s];
         if (!cfast && s == a.nst - 1) {
             mixed_stage_r<R, SIGN, NT, true>(r, src, dst, n, tb, Ns, roots, go);
