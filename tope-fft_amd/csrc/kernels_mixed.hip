@@ -152,14 +152,18 @@ template <typename R>
 struct RowOut {
     const cpx<R> *in;
     cpx<R> *out;
-    long long l0, lines, os, cs;
+    long long l0, lines, os, cs, is;
     Div64 dC;
+    Div32 dtb;
     R scale;
 };
 
 // One stage of compile-time radix r over the block's lines, LDS to LDS,
-// (FROMGLOBAL) the input rows to LDS, or (TOGLOBAL) LDS to the output rows.
-template <typename R, int r, int SIGN, int NT, bool TOGLOBAL, bool FROMGLOBAL = false>
+// (FROMGLOBAL) the input lines to LDS, or (TOGLOBAL) LDS to the output
+// lines.  CF (strided axes, global stages only): neighbouring lanes take
+// neighbouring lines, so a global access of a lane group covers adjacent
+// columns.
+template <typename R, int r, int SIGN, int NT, bool TOGLOBAL, bool FROMGLOBAL = false, bool CF = false>
 __device__ __forceinline__ void mixed_stage(const cpx<R> *src, cpx<R> *dst, int n, int tb, int Ns,
                                             const cpx<R> *__restrict__ roots, const RowOut<R> &go)
 {
@@ -167,8 +171,9 @@ __device__ __forceinline__ void mixed_stage(const cpx<R> *src, cpx<R> *dst, int 
     const int tws = n / (Ns * r);  // table indices stay below n <= 4096: 32-bit
     const Div32 dnb = mkdiv(nb), dns = mkdiv(Ns);
     for (int bi = threadIdx.x; bi < tb * nb; bi += NT) {
-        int j, kk;
-        const int t = divmod(bi, dnb, j);
+        int j, kk, t;
+        if constexpr (CF) j = divmod(bi, go.dtb, t);
+        else t = divmod(bi, dnb, j);
         const int jn = divmod(j, dns, kk);  // j = jn Ns + kk
         cpx<R> x[r];
         if constexpr (FROMGLOBAL) {
@@ -178,7 +183,7 @@ __device__ __forceinline__ void mixed_stage(const cpx<R> *src, cpx<R> *dst, int 
                 const long long o = divmod64(l, go.dC, c);
                 const cpx<R> *row = go.in + o * go.os + c * go.cs;
 #pragma unroll
-                for (int m = 0; m < r; ++m) x[m] = row[j + m * nb];
+                for (int m = 0; m < r; ++m) x[m] = CF ? row[(long long)(j + m * nb) * go.is] : row[j + m * nb];
             } else {
 #pragma unroll
                 for (int m = 0; m < r; ++m) x[m] = cpx<R>{0, 0};
@@ -202,7 +207,11 @@ __device__ __forceinline__ void mixed_stage(const cpx<R> *src, cpx<R> *dst, int 
                 cpx<R> *row = go.out + o * go.os + c * go.cs;
                 const int k0 = jn * Ns * r + kk;
 #pragma unroll
-                for (int m = 0; m < r; ++m) row[k0 + m * Ns] = cpx<R>{x[m].x * go.scale, x[m].y * go.scale};
+                for (int m = 0; m < r; ++m) {
+                    const cpx<R> v{x[m].x * go.scale, x[m].y * go.scale};
+                    if constexpr (CF) row[(long long)(k0 + m * Ns) * go.is] = v;
+                    else row[k0 + m * Ns] = v;
+                }
             }
         } else {
             const int di = t * n + jn * Ns * r + kk;
@@ -212,18 +221,18 @@ __device__ __forceinline__ void mixed_stage(const cpx<R> *src, cpx<R> *dst, int 
     }
 }
 
-template <typename R, int SIGN, int NT, bool TOGLOBAL, bool FROMGLOBAL = false>
+template <typename R, int SIGN, int NT, bool TOGLOBAL, bool FROMGLOBAL = false, bool CF = false>
 __device__ __forceinline__ void mixed_stage_r(int r, const cpx<R> *src, cpx<R> *dst, int n, int tb, int Ns,
                                               const cpx<R> *__restrict__ roots, const RowOut<R> &go)
 {
     switch (r) {
-    case 2: mixed_stage<R, 2, SIGN, NT, TOGLOBAL, FROMGLOBAL>(src, dst, n, tb, Ns, roots, go); break;
-    case 3: mixed_stage<R, 3, SIGN, NT, TOGLOBAL, FROMGLOBAL>(src, dst, n, tb, Ns, roots, go); break;
-    case 4: mixed_stage<R, 4, SIGN, NT, TOGLOBAL, FROMGLOBAL>(src, dst, n, tb, Ns, roots, go); break;
-    case 5: mixed_stage<R, 5, SIGN, NT, TOGLOBAL, FROMGLOBAL>(src, dst, n, tb, Ns, roots, go); break;
-    case 7: mixed_stage<R, 7, SIGN, NT, TOGLOBAL, FROMGLOBAL>(src, dst, n, tb, Ns, roots, go); break;
-    case 8: mixed_stage<R, 8, SIGN, NT, TOGLOBAL, FROMGLOBAL>(src, dst, n, tb, Ns, roots, go); break;
-    default: mixed_stage<R, 16, SIGN, NT, TOGLOBAL, FROMGLOBAL>(src, dst, n, tb, Ns, roots, go); break;
+    case 2: mixed_stage<R, 2, SIGN, NT, TOGLOBAL, FROMGLOBAL, CF>(src, dst, n, tb, Ns, roots, go); break;
+    case 3: mixed_stage<R, 3, SIGN, NT, TOGLOBAL, FROMGLOBAL, CF>(src, dst, n, tb, Ns, roots, go); break;
+    case 4: mixed_stage<R, 4, SIGN, NT, TOGLOBAL, FROMGLOBAL, CF>(src, dst, n, tb, Ns, roots, go); break;
+    case 5: mixed_stage<R, 5, SIGN, NT, TOGLOBAL, FROMGLOBAL, CF>(src, dst, n, tb, Ns, roots, go); break;
+    case 7: mixed_stage<R, 7, SIGN, NT, TOGLOBAL, FROMGLOBAL, CF>(src, dst, n, tb, Ns, roots, go); break;
+    case 8: mixed_stage<R, 8, SIGN, NT, TOGLOBAL, FROMGLOBAL, CF>(src, dst, n, tb, Ns, roots, go); break;
+    default: mixed_stage<R, 16, SIGN, NT, TOGLOBAL, FROMGLOBAL, CF>(src, dst, n, tb, Ns, roots, go); break;
     }
 }
 
@@ -258,18 +267,18 @@ __global__ void __launch_bounds__(NT) mixed_kernel(const cpx<R> *in, cpx<R> *out
         return o * a.os + c * a.cs + (long long)k * a.is;
     };
 
-    // rows of a pass with two or more stages: the first stage reads the
-    // input lines itself (no initial LDS fill and barrier) and the last
-    // stores straight to the output (no final LDS round trip); strided axes
-    // keep both LDS passes so that neighbouring lanes touch neighbouring
-    // lines
-    const RowOut<R> go{in, out, l0, a.lines, a.os, a.cs, dC, (R)a.scale};
-    const bool rows_direct = !cfast && a.nst >= 2;
+    // the first stage of a pass with two or more stages reads the input
+    // lines itself (no initial LDS fill and barrier) and the last stage
+    // stores straight to the output (no final LDS round trip); on strided
+    // axes those two stages put neighbouring lines on neighbouring lanes
+    const RowOut<R> go{in, out, l0, a.lines, a.os, a.cs, a.is, dC, dtb, (R)a.scale};
+    const bool direct_first = a.nst >= 2;
     cpx<R> *src = lds, *dst = lds + SLOTS;
     int Ns = 1;
     int s = 0;
-    if (rows_direct) {
-        mixed_stage_r<R, SIGN, NT, false, true>(a.radix[0], src, dst, n, tb, Ns, roots, go);
+    if (direct_first) {
+        if (cfast) mixed_stage_r<R, SIGN, NT, false, true, true>(a.radix[0], src, dst, n, tb, Ns, roots, go);
+        else mixed_stage_r<R, SIGN, NT, false, true>(a.radix[0], src, dst, n, tb, Ns, roots, go);
         __syncthreads();
         src = lds + SLOTS;
         dst = lds;
@@ -286,8 +295,9 @@ __global__ void __launch_bounds__(NT) mixed_kernel(const cpx<R> *in, cpx<R> *out
     }
     for (; s < a.nst; ++s) {
         const int r = a.radix[s];
-        if (!cfast && s == a.nst - 1) {
-            mixed_stage_r<R, SIGN, NT, true>(r, src, dst, n, tb, Ns, roots, go);
+        if (s == a.nst - 1) {
+            if (cfast) mixed_stage_r<R, SIGN, NT, true, false, true>(r, src, dst, n, tb, Ns, roots, go);
+            else mixed_stage_r<R, SIGN, NT, true>(r, src, dst, n, tb, Ns, roots, go);
             return;
         }
         mixed_stage_r<R, SIGN, NT, false>(r, src, dst, n, tb, Ns, roots, go);
@@ -296,16 +306,6 @@ __global__ void __launch_bounds__(NT) mixed_kernel(const cpx<R> *in, cpx<R> *out
         src = dst;
         dst = tmp;
         Ns *= r;
-    }
-
-    for (int idx = threadIdx.x; idx < span_elems; idx += NT) {
-        int t, k;
-        split(idx, t, k);
-        const long long l = l0 + t;
-        if (l < a.lines) {
-            const cpx<R> v = src[lslot(t * n + k)];
-            out[gaddr(l, k)] = cpx<R>{v.x * (R)a.scale, v.y * (R)a.scale};
-        }
     }
 }
 
