@@ -9,9 +9,10 @@
 // radix r at span Ns treats butterfly j < n/r, reads j + m n/r (m < r),
 // multiplies element m by W_{Ns r}^{(j mod Ns) m} and writes output m of its
 // r-point DFT to (j / Ns) Ns r + (j mod Ns) + m Ns; starting at Ns = 1 the
-// result is in natural order.  A block owns whole lines: it loads them into
-// LDS, runs every stage between two LDS buffers (a barrier per stage), and
-// writes them back, so the pass is in place.  Every twiddle is an entry of
+// result is in natural order.  A block owns whole lines: its first stage
+// reads them from memory, the stages in between run between two LDS
+// buffers (a barrier per stage), and its last stage writes them back, so
+// the pass is in place.  Every twiddle is an entry of
 // the plan's table W_n^i (exactly reduced phases, forward sign) and the
 // radix-3/5/7 constants are correctly rounded literals, so no device sin/cos
 // is evaluated.
