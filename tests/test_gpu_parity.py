@@ -113,7 +113,9 @@ def test_1d_mixed_radix_closed_forms(n):
 
 
 MIXED_ND = [(48, 36), (100, 60), (1000, 7), (3, 729), (1024, 1000), (4000, 3), (12, 10, 14), (60, 9, 25),
-            (7, 256, 5), (100, 11)]
+            (7, 256, 5), (100, 11),
+            # strided lines of 1025..2048 (two or three per block), odd line counts
+            (5, 2000), (3, 1200), (4, 2, 1500), (7, 1125, 2)]
 
 
 @pytest.mark.parametrize("dims", MIXED_ND, ids=lambda d: "x".join(map(str, d)))
@@ -371,7 +373,7 @@ def test_2d_dif2_schedule_small(oracle, monkeypatch, dims):
 
 @pytest.mark.parametrize("dims", [(16, 16), (64, 32), (8, 16, 4), (32, 32, 32), (64, 4, 8),
                                   (12, 10), (100, 6), (15, 7, 9), (1000, 3), (3000, 2),
-                                  (8192, 4), (4, 8192), (16384, 2), (2, 2, 8192)],
+                                  (8192, 4), (4, 8192), (16384, 2), (2, 2, 8192), (5, 2000), (7, 1125, 2)],
                          ids=lambda d: "x".join(map(str, d)))
 def test_nd_complex64(oracle, dims):
     # complex64 storage plans (tope2DPlanInitEx / tope3DPlanInitEx); 7-smooth

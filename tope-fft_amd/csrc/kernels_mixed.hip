@@ -340,6 +340,10 @@ MixedArgs mixed_plan(long long n, long long O, long long os, long long C, long l
     // lines per block: as many as fit 2048 elements (two 32 KiB LDS buffers,
     // two blocks per CU), one line of up to 4096 otherwise
     long long tb = std::max<long long>(1, 2048 / n);
+    // strided lines of 1025..2048 would run one per block (16-byte column
+    // segments): two or three per 4096-element block instead (2048 x 2000:
+    // 89 -> 81.5 us; wider tiles for shorter lines measured slower)
+    if (is != 1 && n > 1024 && n <= 2048) tb = MIXED_MAX_N / n;
     if (tb > a.lines) tb = a.lines;
     a.tb = (int)tb;
     a.scale = 1.0;
