@@ -93,7 +93,8 @@ COLS = [(n, e, tb, True) for n, e, tb in [(2, 2, 64), (4, 4, 64), (8, 8, 64), (1
                                           (128, 16, 16), (256, 16, 16), (512, 16, 8), (1024, 16, 4), (1024, 16, 8),
                                           (2048, 16, 2), (2048, 16, 4), (4096, 16, 1), (4096, 16, 2), (4096, 16, 4),
                                           (4096, 32, 2), (4096, 32, 4), (2048, 32, 4), (2048, 32, 8), (512, 16, 16),
-                                          (512, 16, 32), (256, 16, 32), (256, 16, 64), (256, 16, 4), (256, 4, 1), (512, 8, 1)]]
+                                          (512, 16, 32), (256, 16, 32), (256, 16, 64), (256, 16, 4), (256, 4, 1), (512, 8, 1),
+                                          (1024, 16, 1), (2048, 16, 1)]]
 SHAPES = sorted(set(ROWS + MANY + COLS))
 
 

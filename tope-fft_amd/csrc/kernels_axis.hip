@@ -95,7 +95,8 @@ constexpr int prec_of() { return sizeof(R) == 8 ? PREC_F64 : PREC_F32; }
         VAR(R, 4096, 32, 2, true, false), VAR(R, 4096, 32, 4, true, true), VAR(R, 2048, 32, 4, true, false), \
         VAR(R, 2048, 32, 8, true, true), VAR(R, 512, 16, 16, true, true), VAR(R, 512, 16, 32, true, true),      \
         VAR(R, 256, 16, 32, true, true), VAR(R, 256, 16, 64, true, true), VAR(R, 256, 16, 4, true, false),     \
-        VAR(R, 256, 4, 1, true, false), VAR(R, 512, 8, 1, true, false)
+        VAR(R, 256, 4, 1, true, false), VAR(R, 512, 8, 1, true, false), VAR(R, 1024, 16, 1, true, false), \
+        VAR(R, 2048, 16, 1, true, false)
 
 static const AxisVariant kVariants[] = {ROWS(double), COLS(double), DIF2ROWS(double), ROWS(float), COLS(float)};
 
