@@ -1,14 +1,18 @@
 """Benchmark of the topeFFT transform path on one MI355X.
 
-Workload (one "step"): a forward 2D C2C FP64 transform of a 4096 x 4096
+Workload (one "step"): one 2D C2C FP64 transform of a 4096 x 4096
 complex128 array (256 MiB) already resident in HBM, executed through the
-C-ABI's device-resident entry point (tope2DExecDevice).  Input: seeded
-uniform [-1,1) re/im plus 8 planted unit exponentials.
+C-ABI's device-resident entry point (tope2DExecDevice); the steps alternate
+FORWARD and INVERSE (the same kernels; INVERSE also applies the fused 1/N).
+Input: seeded uniform [-1,1) re/im plus 8 planted unit exponentials.
 
 value = algorithmic GB/s = 2 * 16 B * N * P / t, P = 2 (one HBM read and
 one write per axis), N = 4096^2 -> 1 GiB per transform.  Also printed:
-transforms/s, the same figures for the 3D 256^3 transform, and for the
-dominant kernel its HIP-event duration against the 8 TB/s HBM peak.
+transforms/s, FP64 FLOP/s (5 N log2 N) against the 78.6 TFLOP/s vector
+peak, the same transform in complex64 storage (FP32 VALU and matrix-core
+plans) with its speedup and error against the FP64 result, side workloads,
+and for the dominant kernel its HIP-event duration against the 8 TB/s HBM
+peak.
 
 cpu_baseline: the oracle's CPU restatement of the reference algorithm
 (radix-8 DIT + bit-reversal swap + one full pass per stage, OpenMP) on the
@@ -66,10 +70,13 @@ def kernel_symbol(variant_name, sign=-1):
     return f"tope::axis_kernel<{prec}, {n}, {e}, {tb}, {col}, {split}, {sign}, false"
 
 
+TRAFFIC_FILE = os.path.join("profiles", "pmc_traffic_2d4096_fp64.md")
+
+
 def profiled_traffic(variant_name):
     """HBM bytes per launch of this kernel from the committed rocprofv3
     FETCH_SIZE/WRITE_SIZE summary (scaled by a plain copy), or None."""
-    path = os.path.join(REPO, "profiles", "pmc_traffic_2d4096_fp64.md")
+    path = os.path.join(REPO, TRAFFIC_FILE)
     if not os.path.exists(path):
         return None
     want = kernel_symbol(variant_name)
@@ -99,7 +106,7 @@ def mixed_precision_leg(fw, steps, warmup):
     x = workloads.complex_gaussian(workloads.SEED_BATCH, n * count)
     ref, t64, p64, name64 = measure_batch(fw, x, n, count, topefft.PREC_F64, topefft.MODE_VALU, steps, warmup)
     refn = np.linalg.norm(ref)
-    res = {"workload": "batch 4096 x 1D C2C n=4096, Gaussian(0,1), forward",
+    res = {"workload": "batch 4096 x 1D C2C n=4096, Gaussian(0,1), forward and inverse alternating",
            "fp64": {"kernel": name64, "ms": round(t64 * 1e3, 4), "transforms_per_s": round(count / t64, 1),
                     "gbs": round(gbs(2 * 16 * n * count, t64), 1)},
            "complex64": {}}
@@ -116,6 +123,39 @@ def mixed_precision_leg(fw, steps, warmup):
     return res
 
 
+def mp_2d_leg(fw, x, t64, steps, warmup):
+    """The headline transform in complex64 storage: the FP32 VALU plan and
+    the matrix-core plans (split-fp16 and bf16x6 operands; axes without an
+    MFMA kernel run the FP32 VALU pass, named in `passes`), each timed like
+    the FP64 headline and compared with the FP64 forward spectrum of the
+    same input (rel-L2, max-abs)."""
+    import topefft
+    dims = (4096, 4096)
+    n = dims[0] * dims[1]
+    p = topefft.Plan(fw, dims, x)
+    ref = p.exec(topefft.FORWARD)
+    p.destroy()
+    refn = np.linalg.norm(ref)
+    x32 = x.astype(np.complex64)
+    res = {"workload": "2D C2C 4096x4096 complex64 storage, forward and inverse alternating, device-resident",
+           "fp64_ms": round(t64 * 1e3, 4)}
+    for mode in (None, topefft.MODE_MFMA_FP16X3, topefft.MODE_MFMA_BF16X6):
+        q = topefft.Plan(fw, dims, x32, precision=topefft.PREC_F32, mode=mode)
+        out = q.exec(topefft.FORWARD).astype(np.complex128)
+        q.exec(topefft.INVERSE)
+        t, pm = measure(fw, q, steps, warmup)
+        names = q.pass_names()
+        q.destroy()
+        d = out - ref
+        res["valu" if mode is None else topefft.MODE_NAMES[mode]] = {
+            "ms": round(t * 1e3, 4), "transforms_per_s": round(1.0 / t, 2),
+            "gbs": round(gbs(2 * 8 * n * 2, t), 1), "speedup_vs_fp64": round(t64 / t, 3),
+            "rel_l2_vs_fp64": float(f"{np.linalg.norm(d) / refn:.3e}"),
+            "max_abs_vs_fp64": float(f"{np.max(np.abs(d)):.3e}"),
+            "passes": [{"kernel": nm, "ms": round(float(m), 4)} for nm, m in zip(names, pm)]}
+    return res
+
+
 def uneven_batch_leg(fw, steps, warmup, graphs=True):
     """20,000 transforms, lengths 2^6..2^14 log-uniform, packed with an
     offset table: one persistent launch (direct and hipGraph replay, after
@@ -125,7 +165,7 @@ def uneven_batch_leg(fw, steps, warmup, graphs=True):
     sizes = workloads.uneven_sizes(workloads.SEED_MANY_SIZES, 20000)
     total = int(sizes.sum())
     x = workloads.complex_uniform(workloads.SEED_MANY_DATA, total)
-    res = {"workload": "20000 x 1D C2C, n = 2^6..2^14 log-uniform (seeded), packed, forward",
+    res = {"workload": "20000 x 1D C2C, n = 2^6..2^14 log-uniform (seeded), packed, forward and inverse alternating",
            "elements": total}
     for prec, name, xs, sb in ((topefft.PREC_F64, "fp64", x, 16),
                                (topefft.PREC_F32, "complex64", x.astype(np.complex64), 8)):
@@ -306,7 +346,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="uneven batch: skip the hipGraph-replay methods (for kernel-trace profiling)")
     ap.add_argument("--legs", default="all",
-                    help="comma list of 1d,3d256,mp,many,3d512,r2c (default all; 2d is always run)")
+                    help="comma list of mp2d,1d,3d256,mp,many,3d512,r2c,vendor (default all; 2d is always run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -334,8 +374,11 @@ def main():
     # the side legs (reported beside the headline, not part of `value`) time
     # at most 20 steps each so the default run stays short
     side_steps = max(5, min(20, args.steps // 2))
-    legs = set(args.legs.split(",")) if args.legs != "all" else {"1d", "3d256", "mp", "many", "3d512", "r2c", "vendor"}
+    legs = set(args.legs.split(",")) if args.legs != "all" else {"mp2d", "1d", "3d256", "mp", "many", "3d512", "r2c",
+                                                                  "vendor"}
     extra = {}
+    if "mp2d" in legs:
+        extra["mixed_precision_2d_4096"] = mp_2d_leg(fw, x2, t2, args.steps, args.warmup)
     if "1d" in legs:
         extra["1d_65536"] = small_1d_leg(fw, args.steps, args.warmup)
     # ---- 3D 256^3 FP64 (same recipe, beside it) ---------------------------
@@ -387,15 +430,18 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic: seeded uniform [-1,1) re/im + 8 planted unit exponentials",
-        "config": {"workload": "2D C2C FP64 4096x4096 complex128 forward, device-resident",
+        "config": {"workload": "2D C2C FP64 4096x4096 complex128, forward and inverse alternating, device-resident",
                    "transforms_per_s": round(1.0 / t2, 2),
                    "fp64_tflops_5nlogn": round(flops2 / t2 / 1e12, 3),
+                   "fp64_flops_frac": round(flops2 / t2 / 1e12 / FP64_VECTOR_PEAK_TFLOPS, 4),
                    "algorithmic_bytes_per_transform": alg2,
                    "passes": {n: round(float(m), 4) for n, m in zip(names2, pass2)},
                    **extra},
         "roofline": {"bound": "hbm", "kernel": names2[dom], "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": profiled_traffic(names2[dom]),
+                     "traffic_source": f"{TRAFFIC_FILE}: rocprofv3 FETCH_SIZE and WRITE_SIZE of this kernel, "
+                                       "each scaled by what it reads for a plain copy (a separate profiled run)",
                      "per_launch_algorithmic_bytes": dom_bytes,
                      "avg_launch_ms": round(float(pass2[dom]), 5)},
         "cpu_baseline": cpu,

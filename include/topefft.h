@@ -379,6 +379,11 @@ const char *topeErrorString(int code);
  * FP64 twiddle generation against exact values. */
 long long topePlanTwiddles(struct topeFFT *f, void *plan_impl, int axis, double *host, long long max_entries);
 
+/* Name of the index-th compiled axis-pass kernel variant (the names
+ * topePlanPassName reports and TOPEFFT_VARIANT_<n>_<row|col>=<name>
+ * selects), or NULL past the last one.  Lets tests run every variant. */
+const char *topeAxisVariantName(int index);
+
 #ifdef __cplusplus
 }
 #endif

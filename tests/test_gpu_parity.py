@@ -389,29 +389,6 @@ def test_nd_complex64(oracle, dims):
     assert rel_l2(back, x64) < 1e-6
 
 
-def test_3d_512_fp64_and_complex64():
-    # the 512^3 configuration: FP64 forward then inverse round trip, and the
-    # complex64 plan against the FP64 forward result
-    dims = (512, 512, 512)
-    n = 512 ** 3
-    x = workloads.complex_uniform(workloads.SEED_3D_LARGE, n)
-    fw = topefft.Framework()
-    p = topefft.Plan(fw, dims, x)
-    X = p.exec(topefft.FORWARD)
-    back = p.exec(topefft.INVERSE)
-    p.destroy()
-    assert rel_l2(back, x) < TOL
-    # Parseval on the FP64 spectrum
-    assert abs(np.vdot(X, X).real - n * np.vdot(x, x).real) / (n * np.vdot(x, x).real) < TOL
-    fw = topefft.Framework()
-    q = topefft.Plan(fw, dims, x.astype(np.complex64), precision=topefft.PREC_F32)
-    X32 = q.exec(topefft.FORWARD)
-    q.destroy()
-    err = rel_l2(X32.astype(np.complex128), X)
-    print(f"512^3 complex64 vs FP64 rel-L2 {err:.3e}")
-    assert err < 1e-6
-
-
 def test_3d_256_row_tiles_complex64_against_fp64():
     # 256^3 takes the 4-row tiles for its 256-point rows (shapes of at least
     # 8192 such blocks), in FP64 and complex64; the complex64 plan agrees

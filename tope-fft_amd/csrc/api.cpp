@@ -229,8 +229,9 @@ const AxisVariant *select_variant(int prec, int n, bool cfast, long long lines, 
     }
     int cnt = 0;
     const AxisVariant *vs = axis_variants(&cnt);
+    // the even/odd-split tiles compete only through the preference table
     auto matches = [&](const AxisVariant *v) {
-        return v->prec == prec && v->n == n && v->cfast == cfast && v->dif2 == dif2;
+        return v->prec == prec && v->n == n && v->cfast == cfast && v->dif2 == dif2 && !v->dit2;
     };
     // Small shapes: when no tile gives every CU a block, take the tile that
     // gives the most blocks (full-complex exchange first); otherwise only
@@ -363,14 +364,23 @@ class Planner {
         const bool cfast = !(gi.is == 1 && go.is == 1);
         const AxisVariant *v = select_variant(P->prec, n, cfast, dif2 ? 2 * gi.C : gi.C, dif2, gi.O * reps);
         if (!v) return TOPE_ERROR_NO_KERNEL;
-        HostSched h = host_sched(v->n, v->e);
+        // even/odd-split tiles run the stages of an n/2-point transform and
+        // combine the halves with W_n^k, k < n/2 (passed as dif_tw)
+        const int tn = v->dit2 ? v->n / 2 : v->n;
+        HostSched h = host_sched(tn, v->e);
         if (h.twsize != v->twsize) return TOPE_ERROR_NO_KERNEL;  // host/device layouts disagree
         void *tw = nullptr;
         // (kind stays PASS_AXIS)
-        hipError_t err = stage_table(v->n, v->e, &tw);
+        hipError_t err = stage_table(tn, v->e, &tw);
         if (err != hipSuccess) return map_err(err);
+        if (v->dit2) {
+            void *wk = nullptr;
+            err = power_table(v->n, 1, v->n / 2, &wk);
+            if (err != hipSuccess) return map_err(err);
+            dif_tw = wk;
+        }
         if (axis >= 0 && !P->axis_tw[axis].table) {
-            P->axis_tw[axis].n = v->n;
+            P->axis_tw[axis].n = tn;
             P->axis_tw[axis].e = v->e;
             P->axis_tw[axis].table = tw;
         }
@@ -1252,6 +1262,10 @@ void fw_release(struct topeFFT *f, void *owner)
     if (fw && fw_of(f) == fw && --fw->live_plans <= 0) release_framework(f);
 }
 const AxisVariant *row_variant(int prec, int n) { return select_variant(prec, n, false, 1); }
+const AxisVariant *axis_variant_for(int prec, int n, bool cfast, long long lines, long long outer)
+{
+    return select_variant(prec, n, cfast, lines, false, outer);
+}
 }  // namespace tope
 
 extern "C" {
@@ -1942,6 +1956,13 @@ const char *topeErrorString(int code)
     case TOPE_ERROR_UNSUPPORTED: return "unsupported request";
     default: return code > 0 ? hipGetErrorString((hipError_t)code) : "unknown error";
     }
+}
+
+const char *topeAxisVariantName(int index)
+{
+    int cnt = 0;
+    const AxisVariant *v = axis_variants(&cnt);
+    return (index >= 0 && index < cnt) ? v[index].name : nullptr;
 }
 
 long long topePlanTwiddles(struct topeFFT *f, void *plan_impl, int axis, double *host, long long max_entries)

@@ -24,8 +24,10 @@ struct AxisVariant {
     bool cfast;   // lanes walk transforms first (strided-axis tiles)
     bool split;   // real/imag LDS exchange in two rounds
     bool dif2;    // row pairs + radix-2 DIF step of the next axis
+    bool dit2;    // column tiles split into even/odd halves (axis_dit2.hpp):
+                  // stage table of (n/2, e), W_n^k table (k < n/2) in dif_tw
     int threads;  // block size
-    int twsize;   // stage twiddle entries for (n, e)
+    int twsize;   // stage twiddle entries for (n, e), (n/2, e) when dit2
     int lds_bytes;
     AxisLaunchFn fn;
     const char *name;
@@ -116,6 +118,9 @@ void fw_retain(struct topeFFT *f);
 void fw_release(struct topeFFT *f, void *owner);
 // Row variant the planner would pick for one transform of length n.
 const AxisVariant *row_variant(int prec, int n);
+// Variant the planner would pick for `lines` transforms of length n (per
+// outer index, `outer` of them); cfast: lanes walk the transforms first.
+const AxisVariant *axis_variant_for(int prec, int n, bool cfast, long long lines, long long outer);
 
 // Persistent uneven batch (many_kernel.hpp); args is ManyArgs<double|float>.
 hipError_t launch_many(int prec, int threads, const void *in, void *out, const void *args, int grid, int sign,

@@ -7,6 +7,7 @@
 // instruction covers 64/TB rows of TB contiguous elements (TB*16 bytes for
 // FP64); the split-exchange column variants trade two extra barriers per
 // stage for twice as many columns per block.
+#include "axis_dit2.hpp"
 #include "internal.hpp"
 
 namespace tope {
@@ -38,10 +39,33 @@ static void launch_axis_t(const void *in, void *out, const void *args, unsigned 
                            i, o, a);
 }
 
+// Even/odd-split column tiles (dit2_kernel): NT = TB * (N/2) / E threads.
+template <typename R, int N, int E, int TB, bool SPLIT, bool BIG>
+static void launch_dit2_t(const void *in, void *out, const void *args, unsigned grid, hipStream_t s, int sign)
+{
+    constexpr int NT = TB * (N / 2 / E);
+    const AxisArgs<R> &a = *static_cast<const AxisArgs<R> *>(args);
+    const cpx<R> *i = static_cast<const cpx<R> *>(in);
+    cpx<R> *o = static_cast<cpx<R> *>(out);
+    if (a.pt_n) {
+        if (sign < 0)
+            hipLaunchKernelGGL((dit2_kernel<R, N, E, TB, SPLIT, -1, true, BIG>), dim3(grid), dim3(NT), 0, s, i, o, a);
+        else
+            hipLaunchKernelGGL((dit2_kernel<R, N, E, TB, SPLIT, 1, true, BIG>), dim3(grid), dim3(NT), 0, s, i, o, a);
+    } else {
+        if (sign < 0)
+            hipLaunchKernelGGL((dit2_kernel<R, N, E, TB, SPLIT, -1, false, BIG>), dim3(grid), dim3(NT), 0, s, i, o, a);
+        else
+            hipLaunchKernelGGL((dit2_kernel<R, N, E, TB, SPLIT, 1, false, BIG>), dim3(grid), dim3(NT), 0, s, i, o, a);
+    }
+}
+
 #ifdef TOPE_AXIS_BIG
 #define TOPE_LAUNCH(R, N, E, TB, CF, SP, DF) &launch_axis_t<R, N, E, TB, CF, SP, true, DF>
+#define TOPE_LAUNCH_DIT2(R, N, E, TB, SP) &launch_dit2_t<R, N, E, TB, SP, true>
 #else
 #define TOPE_LAUNCH(R, N, E, TB, CF, SP, DF) &launch_axis_t<R, N, E, TB, CF, SP, false, DF>
+#define TOPE_LAUNCH_DIT2(R, N, E, TB, SP) &launch_dit2_t<R, N, E, TB, SP, false>
 #endif
 
 template <typename R>
@@ -56,11 +80,19 @@ constexpr int prec_of() { return sizeof(R) == 8 ? PREC_F64 : PREC_F32; }
 #define TOPE_STR2(x) #x
 #define TOPE_STR(x) TOPE_STR2(x)
 #define VARD(R, N, E, TB, CF, SP, DF)                                                                        \
-    AxisVariant{prec_of<R>(), N, E, TB, CF, SP, DF, TB * (N / E), Sched<N, E>::TWSIZE,                      \
+    AxisVariant{prec_of<R>(), N, E, TB, CF, SP, DF, false, TB * (N / E), Sched<N, E>::TWSIZE,               \
                 Exchange<R, N, E, TB, CF, SP>::BYTES, TOPE_LAUNCH(R, N, E, TB, CF, SP, DF),                \
                 "axis_" TOPE_STR(R) "_n" TOPE_STR(N) "_e" TOPE_STR(E) "_tb" TOPE_STR(TB) TOPE_LAYOUT_##CF \
                     TOPE_SPLIT_##SP TOPE_DIF2_##DF}
 #define VAR(R, N, E, TB, CF, SP) VARD(R, N, E, TB, CF, SP, false)
+#define VARH(R, N, E, TB, SP)                                                                                \
+    AxisVariant{prec_of<R>(), N, E, TB, true, SP, false, true, TB * (N / 2 / E), Sched<N / 2, E>::TWSIZE,  \
+                Dit2Lds<R, N, E, TB, SP>::BYTES, TOPE_LAUNCH_DIT2(R, N, E, TB, SP),                        \
+                "axis_" TOPE_STR(R) "_n" TOPE_STR(N) "_e" TOPE_STR(E) "_tb" TOPE_STR(TB) "_col" TOPE_SPLIT_##SP "_dit2"}
+
+// Even/odd-split column tiles for the long strided axes.
+#define DIT2COLS(R) VARH(R, 4096, 8, 4, true), VARH(R, 4096, 16, 4, true), VARH(R, 4096, 16, 4, false), \
+    VARH(R, 4096, 8, 4, false)
 
 // Row-pair variants (2D passes whose next axis takes a fused radix-2 DIF
 // step): the row table's tiles, at least two rows, split exchange.
@@ -98,7 +130,8 @@ constexpr int prec_of() { return sizeof(R) == 8 ? PREC_F64 : PREC_F32; }
         VAR(R, 256, 4, 1, true, false), VAR(R, 512, 8, 1, true, false), VAR(R, 1024, 16, 1, true, false), \
         VAR(R, 2048, 16, 1, true, false)
 
-static const AxisVariant kVariants[] = {ROWS(double), COLS(double), DIF2ROWS(double), ROWS(float), COLS(float)};
+static const AxisVariant kVariants[] = {ROWS(double), COLS(double), DIF2ROWS(double), ROWS(float), COLS(float),
+                                        DIT2COLS(double)};
 
 // The same list is compiled twice: normally (32-bit buffer offsets) and with
 // TOPE_AXIS_BIG (64-bit pointers) for blocks spanning more than 2^31 bytes.

@@ -143,6 +143,7 @@ SIGNATURES = {
     "topeManyDestroy": (None, [_F, _PM]),
     "topeErrorString": (C.c_char_p, [C.c_int]),
     "topePlanTwiddles": (C.c_longlong, [_F, C.c_void_p, C.c_int, _DP, C.c_longlong]),
+    "topeAxisVariantName": (C.c_char_p, [C.c_int]),
 }
 
 _lib = None
@@ -527,3 +528,22 @@ def transform(x, dims, direction=FORWARD):
     finally:
         p.destroy()
         fw.destroy()
+
+
+def axis_variants():
+    """Names of every compiled axis-pass kernel variant, parsed into
+    (name, precision, n, e, tb, col, split, dif2)."""
+    lib = load()
+    out = []
+    i = 0
+    while True:
+        nm = lib.topeAxisVariantName(i)
+        if nm is None:
+            break
+        nm = nm.decode()
+        parts = nm.split("_")
+        prec = PREC_F64 if parts[1] == "double" else PREC_F32
+        out.append((nm, prec, int(parts[2][1:]), int(parts[3][1:]), int(parts[4][2:]), parts[5] == "col",
+                    "split" in parts[6:], "dif2" in parts[6:]))
+        i += 1
+    return out
