@@ -62,11 +62,14 @@ def measure(fw, plan, steps, warmup):
 
 def kernel_symbol(variant_name, sign=-1):
     """axis_double_n4096_e16_tb4_col_split -> the kernel's template name
-    prefix as rocprofv3 prints it (forward, no post-twiddle)."""
+    prefix as rocprofv3 prints it (forward, no post-twiddle); the
+    even/odd-split column tiles (..._dit2) are dit2_kernel."""
     parts = variant_name.split("_")
     prec, n, e, tb = parts[1], parts[2][1:], parts[3][1:], parts[4][2:]
     col = "true" if parts[5] == "col" else "false"
-    split = "true" if parts[-1] == "split" else "false"
+    split = "true" if "split" in parts[6:] else "false"
+    if "dit2" in parts[6:]:
+        return f"tope::dit2_kernel<{prec}, {n}, {e}, {tb}, {split}, {sign}, false"
     return f"tope::axis_kernel<{prec}, {n}, {e}, {tb}, {col}, {split}, {sign}, false"
 
 
@@ -187,9 +190,13 @@ def uneven_batch_leg(fw, steps, warmup, graphs=True):
                 p.exec_device(dirs[i % 2])
             fw.synchronize()
             t = (time.perf_counter() - t0) / k
+            # device time per exec from HIP events on the plan's stream (the
+            # graph methods: per replay of the captured graph)
+            dev_ms = p.time_exec(k)
             entry[topefft.MANY_METHOD_NAMES[m]] = {"ms": round(t * 1e3, 4),
                                                     "gbs": round(gbs(2 * sb * total, t), 1),
-                                                    "transforms_per_s": round(len(sizes) / t, 1)}
+                                                    "transforms_per_s": round(len(sizes) / t, 1),
+                                                    "device_ms_hip_events": round(dev_ms, 4)}
         p.destroy()
         res[name] = entry
     return res

@@ -22,8 +22,9 @@
  *     src/kernels1D.cl:549-573): 7-smooth lengths (factors 2, 3, 5, 7) up
  *     to 4096 by a mixed-radix pass, other lengths by a GPU DFT up to 64
  *     and Bluestein's chirp-z convolution above; 2D/3D axes that are not
- *     powers of two (7-smooth up to 4096, others up to 2048) run the same
- *     way where the reference exits (src/fft2d.c:355-362).
+ *     powers of two (7-smooth up to 4096 by the mixed-radix pass, every
+ *     other length up to 2^23 by Bluestein) run the same way where the
+ *     reference exits (src/fft2d.c:355-362).
  *   - the n == 2 1D inverse divides both real and imaginary parts by 2
  *     (src/fft1d.c:23-26 divides only the real parts).
  *   - tope*Destroy frees the plan; the framework (stream) is released when
@@ -203,7 +204,7 @@ const char *topePlanPassName(void *plan_impl, int pass);
  * MFMA modes take n = 256, 512, 1024, 2048 or 4096.  Twiddles and DFT-matrix constants are
  * generated in FP64 on the GPU and rounded once; INVERSE scales by 1/n.
  * Both precisions take powers of two and 7-smooth n up to 4096 (mixed-radix
- * pass, VALU mode); FP64 plans also take other n up to 2048 (Bluestein). */
+ * pass, VALU mode); FP64 plans also take other n up to 2^23 (Bluestein). */
 #define TOPE_PREC_F64 0
 #define TOPE_PREC_F32 1
 #define TOPE_MODE_VALU 0
@@ -360,6 +361,12 @@ void topeManyExecDevice(struct topeFFT *f, struct topePlanMany *t, int dir);
  * plan's stream; the graph methods run their launches without the graph
  * (the graphs hold the plan's own array). */
 void topeManyExecPtr(struct topeFFT *f, struct topePlanMany *t, void *data, int dir);
+/* topeManyTimeExec: mean device time in milliseconds of one exec of the
+ * plan's current method, from HIP events around `reps` back-to-back execs
+ * on the plan's stream (directions alternating, FORWARD first; the graph
+ * methods replay their graphs), after one untimed exec.  Transforms the
+ * plan's array. */
+double topeManyTimeExec(struct topeFFT *f, struct topePlanMany *t, int reps);
 long long topeManyOffsets(struct topePlanMany *t, long long *offsets);
 void topeManyGetSchedule(struct topePlanMany *t, int *radix_by_log2, int *grid);
 int topeManyTune(struct topeFFT *f, struct topePlanMany *t, const char *wisdom_path, int reps);

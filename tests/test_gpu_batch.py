@@ -150,8 +150,24 @@ def test_batch_rejects_unsupported():
     with pytest.raises(topefft.TopeError) as e:
         topefft.BatchPlan(fw, 100, 2, np.zeros(200, np.complex64), topefft.PREC_F32, topefft.MODE_MFMA_FP16X3)
     assert e.value.code == topefft.TOPE_ERROR_UNSUPPORTED
-    # Bluestein rows of M = 2^k >= 2n - 1 up to 4096
+    # Bluestein rows of M = 2^k >= 2n - 1 up to 2^24 points
+    n = (1 << 23) + 3
     with pytest.raises(topefft.TopeError) as e:
-        topefft.BatchPlan(fw, 2053, 2, np.zeros(4106, complex), topefft.PREC_F64, topefft.MODE_VALU)
+        topefft.BatchPlan(fw, n, 1, np.zeros(n, complex), topefft.PREC_F64, topefft.MODE_VALU)
     assert e.value.code == topefft.TOPE_ERROR_INVALID_SIZE
     fw.destroy()
+
+
+def test_batch_long_bluestein(oracle):
+    # FP64 batches of non-power-of-two lengths whose Bluestein rows are
+    # longer than one pass (M = 8192: the inner FFTs run the four-step)
+    n, count = 2053, 3
+    rng = np.random.default_rng(2053)
+    x = rng.standard_normal(n * count) + 1j * rng.standard_normal(n * count)
+    fw = topefft.Framework()
+    p = topefft.BatchPlan(fw, n, count, x, topefft.PREC_F64, topefft.MODE_VALU)
+    X = p.exec(topefft.FORWARD)
+    p.destroy()
+    for i in range(count):
+        seg = slice(i * n, (i + 1) * n)
+        assert rel_l2(X[seg], oracle.dft(x[seg], (n,), topefft.FORWARD)) < 1e-12

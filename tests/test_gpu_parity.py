@@ -328,18 +328,40 @@ def test_profile_counters_and_passes():
 
 def test_error_paths():
     gfw = topefft.Framework()
-    # 2D/3D axes that are not powers of two: 7-smooth up to 4096 (mixed
-    # radix), others up to 2048 (Bluestein)
-    for n in (3001, 5000):
-        with pytest.raises(topefft.TopeError) as e:
-            topefft.Plan(gfw, (n, 2), np.zeros(n * 2, complex))
-        assert e.value.code == topefft.TOPE_ERROR_INVALID_SIZE, n
+    # a 2D/3D axis longer than 2^23 that is not a power of two: its
+    # Bluestein length would exceed 2^24
+    n = (1 << 23) + 1
+    with pytest.raises(topefft.TopeError) as e:
+        topefft.Plan(gfw, (n, 1), np.zeros(n, complex))
+    assert e.value.code == topefft.TOPE_ERROR_INVALID_SIZE, n
     p = topefft.Plan(gfw, (16,), np.zeros(16, complex))
     with pytest.raises(topefft.TopeError) as e:
         p.exec(5)
     assert e.value.code == topefft.TOPE_ERROR_INVALID_ARG
     p.destroy()
     gfw.destroy()
+
+
+# the reference exits on these (src/fft2d.c:355-362, src/fft3d.c:522-533):
+# 7-smooth axes above 4096 and other axes above 2048 run Bluestein with an
+# inner FFT of 8192 or 16384 points (four-step)
+LONG_NON_POW2 = [(6000, 4), (4, 5000), (3001, 2), (4097, 3), (2, 4100, 3), (3, 2, 4099)]
+
+
+@pytest.mark.parametrize("dims", LONG_NON_POW2, ids=lambda d: "x".join(map(str, d)))
+def test_nd_long_non_pow2(oracle, dims):
+    x = seeded(dims, salt=21)
+    fw = topefft.Framework()
+    p = topefft.Plan(fw, dims, x)
+    names = p.pass_names()
+    fwd = p.exec(topefft.FORWARD)
+    back = p.exec(topefft.INVERSE)
+    p.destroy()
+    fw.destroy()
+    assert "chirp_pre" in names, names
+    assert rel_l2(fwd, oracle.dft(x, dims, topefft.FORWARD)) < TOL, dims
+    assert rel_l2(back, x) < TOL, dims
+    assert rel_l2(topefft.transform(x, dims, topefft.INVERSE), oracle.dft(x, dims, topefft.INVERSE)) < TOL
 
 
 def test_destroy_last_plan_releases_framework():

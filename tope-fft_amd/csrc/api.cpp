@@ -561,7 +561,8 @@ class Planner {
             P->passes.push_back(ps);
         };
         // the rows live in the scratch buffer; an inner FFT longer than one
-        // pass (1D only: one row) uses the data buffer, consumed by then
+        // pass uses the data buffer as its intermediate: the pre-multiply
+        // has read every line of the axis (the whole array) by then
         chirp_pass(CHIRP_PRE, data, scratch);
         int rc_ = scratch;
         const size_t f0 = P->passes.size();
@@ -899,8 +900,10 @@ int build_plan(struct topeFFT *f, int shape, long long nx, long long ny, long lo
         for (int ax = 0; ax < (shape == SHAPE_BATCH ? 1 : 3); ++ax) {
             const long long n = dims3[ax];
             if (n == 1 || is_pow2(n) || mixed_supported(n) || (shape == 1 && n <= kChirpMin)) continue;
+            // M above 4096 runs its FFTs as the four-step, through the
+            // data buffer (consumed by the chirp pre-multiply by then)
             const long long M = Planner::chirp_len(n);
-            if (M > (shape == 1 ? kMaxAxis : (long long)kMaxFused)) {
+            if (M > kMaxAxis) {
                 delete P;
                 return TOPE_ERROR_INVALID_SIZE;
             }

@@ -131,7 +131,8 @@ constexpr int prec_of() { return sizeof(R) == 8 ? PREC_F64 : PREC_F32; }
         VAR(R, 2048, 16, 1, true, false)
 
 static const AxisVariant kVariants[] = {ROWS(double), COLS(double), DIF2ROWS(double), ROWS(float), COLS(float),
-                                        DIT2COLS(double)};
+                                        DIT2COLS(double), VARH(float, 4096, 16, 8, true),
+                                        VARH(float, 4096, 16, 8, false)};
 
 // The same list is compiled twice: normally (32-bit buffer offsets) and with
 // TOPE_AXIS_BIG (64-bit pointers) for blocks spanning more than 2^31 bytes.

@@ -138,6 +138,7 @@ SIGNATURES = {
     "topeManyExecDevice": (None, [_F, _PM, C.c_int]),
     "topeManyExecPtr": (None, [_F, _PM, C.c_void_p, C.c_int]),
     "topeManyOffsets": (C.c_longlong, [_PM, C.POINTER(C.c_longlong)]),
+    "topeManyTimeExec": (C.c_double, [_F, _PM, C.c_int]),
     "topeManyGetSchedule": (None, [_PM, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "topeManyTune": (C.c_int, [_F, _PM, C.c_char_p, C.c_int]),
     "topeManyDestroy": (None, [_F, _PM]),
@@ -500,6 +501,12 @@ class ManyPlan:
         place, asynchronously on the plan's stream."""
         self.lib.topeManyExecPtr(C.byref(self.fw.f), C.byref(self.t), C.c_void_p(device_ptr), direction)
         self.fw.check("topeManyExecPtr")
+
+    def time_exec(self, reps=10):
+        """Mean device ms of one exec of the current method (HIP events)."""
+        ms = self.lib.topeManyTimeExec(C.byref(self.fw.f), C.byref(self.t), reps)
+        self.fw.check("topeManyTimeExec")
+        return ms
 
     def schedule(self):
         r = (C.c_int * 16)()
