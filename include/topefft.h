@@ -63,6 +63,7 @@ extern "C" {
 #define TOPE_ERROR_NOT_INITIALIZED (-1004)
 #define TOPE_ERROR_NO_KERNEL (-1005)
 #define TOPE_ERROR_UNSUPPORTED (-1006)
+#define TOPE_ERROR_SCRATCH_LIMIT (-1007)
 
 typedef uint64_t tope_ulong; /* was cl_ulong */
 
@@ -385,6 +386,22 @@ const char *topeErrorString(int code);
  * with host == NULL only the count is returned.  Lets tests pin the GPU's
  * FP64 twiddle generation against exact values. */
 long long topePlanTwiddles(struct topeFFT *f, void *plan_impl, int axis, double *host, long long max_entries);
+
+/* ---- allocation-free transforms (no reference counterpart) -------------
+ * For callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.
+ * topeExec1DNoAlloc transforms `count` length-n complex128 transforms stored
+ * back to back in the caller's device array `data`, in place, on f->stream,
+ * without a plan and without allocating: it runs the kernels of a 1D plan's
+ * topePlanExecPtr, with their twiddle tables and (n > 4096) the four-step's
+ * intermediate array in a static device array of the library (one per GPU,
+ * held until the process ends).  n: power of two 1 .. 2^24; for n > 4096,
+ * count * n <= TOPE_NOALLOC_MAX_ELEMENTS, else f->error =
+ * TOPE_ERROR_SCRATCH_LIMIT.  The static array is shared by every call on a
+ * GPU: calls that overlap on one GPU (other streams, other host threads)
+ * race.  FORWARD unnormalised, INVERSE scaled by 1/n.  Asynchronous. */
+#define TOPE_NOALLOC_MAX_ELEMENTS (1 << 21)
+void topeExec1DNoAlloc(struct topeFFT *f, void *data, int n, int count, int dir);
 
 /* Name of the index-th compiled axis-pass kernel variant (the names
  * topePlanPassName reports and TOPEFFT_VARIANT_<n>_<row|col>=<name>

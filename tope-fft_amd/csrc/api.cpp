@@ -1296,6 +1296,15 @@ void topeFFTInit(struct topeFFT *f)
         f->error = map_err(err);
         return;
     }
+    // device address of the allocation-free entry point's static scratch
+    // array on this GPU (once per device)
+    err = noalloc_setup(dev);
+    if (err != hipSuccess) {
+        (void)hipStreamDestroy(fw->own);
+        delete fw;
+        f->error = map_err(err);
+        return;
+    }
     fw->stream = fw->own;
     f->impl = fw;
     f->stream = fw->stream;
@@ -1957,8 +1966,23 @@ const char *topeErrorString(int code)
     case TOPE_ERROR_NOT_INITIALIZED: return "framework not initialised (topeFFTInit)";
     case TOPE_ERROR_NO_KERNEL: return "no compiled kernel for this shape";
     case TOPE_ERROR_UNSUPPORTED: return "unsupported request";
+    case TOPE_ERROR_SCRATCH_LIMIT: return "transform exceeds the allocation-free entry point's scratch array";
     default: return code > 0 ? hipGetErrorString((hipError_t)code) : "unknown error";
     }
+}
+
+void topeExec1DNoAlloc(struct topeFFT *f, void *data, int n, int count, int dir)
+{
+    FrameworkImpl *fw = fw_of(f);
+    if (!fw) {
+        if (f) f->error = TOPE_ERROR_NOT_INITIALIZED;
+        return;
+    }
+    if (!data || (dir != FORWARD && dir != INVERSE)) {
+        f->error = TOPE_ERROR_INVALID_ARG;
+        return;
+    }
+    f->error = noalloc_exec_1d(f->device, data, n, count, dir == INVERSE, fw->stream);
 }
 
 const char *topeAxisVariantName(int index)

@@ -122,6 +122,13 @@ const AxisVariant *row_variant(int prec, int n);
 // outer index, `outer` of them); cfast: lanes walk the transforms first.
 const AxisVariant *axis_variant_for(int prec, int n, bool cfast, long long lines, long long outer);
 
+// Allocation-free entry point (noalloc.hip): per-device address of the
+// static scratch array (taken once per device at topeFFTInit), and the
+// plan-free 1D launch path over it (returns a TOPE status).
+hipError_t noalloc_setup(int dev);
+void *noalloc_ptr(int dev);
+int noalloc_exec_1d(int dev, void *data, long long n, long long count, int inverse, hipStream_t s);
+
 // Persistent uneven batch (many_kernel.hpp); args is ManyArgs<double|float>.
 hipError_t launch_many(int prec, int threads, const void *in, void *out, const void *args, int grid, int sign,
                        hipStream_t s);

@@ -21,6 +21,8 @@ TOPE_ERROR_NO_DEVICE = -1003
 TOPE_ERROR_NOT_INITIALIZED = -1004
 TOPE_ERROR_NO_KERNEL = -1005
 TOPE_ERROR_UNSUPPORTED = -1006
+TOPE_ERROR_SCRATCH_LIMIT = -1007
+NOALLOC_MAX_ELEMENTS = 1 << 21
 
 
 class TopeFFT(C.Structure):
@@ -145,6 +147,7 @@ SIGNATURES = {
     "topeErrorString": (C.c_char_p, [C.c_int]),
     "topePlanTwiddles": (C.c_longlong, [_F, C.c_void_p, C.c_int, _DP, C.c_longlong]),
     "topeAxisVariantName": (C.c_char_p, [C.c_int]),
+    "topeExec1DNoAlloc": (None, [_F, C.c_void_p, C.c_int, C.c_int, C.c_int]),
 }
 
 _lib = None
