@@ -210,19 +210,22 @@ const Preferred kPreferred[] = {
 // (wider tiles give longer contiguous segments per row), preferring the
 // full-complex exchange at equal width.  TOPEFFT_VARIANT_<n>_<row|col>=<name>
 // overrides (used by tuning sweeps).
+// plain_only: one-shot variants only (no even/odd split, which needs an
+// extra table), for callers that provide no tables of their own.
 const AxisVariant *select_variant(int prec, int n, bool cfast, long long lines, bool dif2 = false,
-                                  long long outer = 1)
+                                  long long outer = 1, bool plain_only = false)
 {
     char key[64];
     std::snprintf(key, sizeof key, "TOPEFFT_VARIANT_%d_%s%s", n, cfast ? "col" : "row", dif2 ? "_dif2" : "");
     if (const char *ov = std::getenv(key)) {
         const AxisVariant *v = find_variant_named(ov);
-        if (v && v->prec == prec && v->n == n && v->dif2 == dif2) return v;
+        if (v && v->prec == prec && v->n == n && v->dif2 == dif2 && !(plain_only && v->dit2)) return v;
     }
     // blocks a tile width gives for this shape (small shapes: keep the CUs busy)
     auto blocks = [&](const AxisVariant *v) { return outer * ((lines + v->tb - 1) / v->tb); };
     for (const Preferred &pr : kPreferred) {
         const AxisVariant *v = find_variant_named(pr.name);
+        if (v && plain_only && v->dit2) continue;
         if (v && v->prec == prec && v->n == n && v->cfast == cfast && v->dif2 == dif2 && v->tb <= lines &&
             blocks(v) >= pr.min_blocks)
             return v;
@@ -1267,7 +1270,7 @@ void fw_release(struct topeFFT *f, void *owner)
 const AxisVariant *row_variant(int prec, int n) { return select_variant(prec, n, false, 1); }
 const AxisVariant *axis_variant_for(int prec, int n, bool cfast, long long lines, long long outer)
 {
-    return select_variant(prec, n, cfast, lines, false, outer);
+    return select_variant(prec, n, cfast, lines, false, outer, true);
 }
 }  // namespace tope
 

@@ -118,8 +118,9 @@ void fw_retain(struct topeFFT *f);
 void fw_release(struct topeFFT *f, void *owner);
 // Row variant the planner would pick for one transform of length n.
 const AxisVariant *row_variant(int prec, int n);
-// Variant the planner would pick for `lines` transforms of length n (per
-// outer index, `outer` of them); cfast: lanes walk the transforms first.
+// One-shot variant the planner would pick for `lines` transforms of length
+// n (per outer index, `outer` of them); cfast: lanes walk the transforms
+// first.  Never an even/odd-split variant.
 const AxisVariant *axis_variant_for(int prec, int n, bool cfast, long long lines, long long outer);
 
 // Allocation-free entry point (noalloc.hip): per-device address of the

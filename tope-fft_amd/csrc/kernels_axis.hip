@@ -132,7 +132,7 @@ constexpr int prec_of() { return sizeof(R) == 8 ? PREC_F64 : PREC_F32; }
 
 static const AxisVariant kVariants[] = {ROWS(double), COLS(double), DIF2ROWS(double), ROWS(float), COLS(float),
                                         DIT2COLS(double), VARH(float, 4096, 16, 8, true),
-                                        VARH(float, 4096, 16, 8, false)};
+                                        VARH(float, 4096, 16, 8, false), VAR(float, 4096, 32, 8, true, true)};
 
 // The same list is compiled twice: normally (32-bit buffer offsets) and with
 // TOPE_AXIS_BIG (64-bit pointers) for blocks spanning more than 2^31 bytes.

@@ -3,6 +3,8 @@
   python tope-fft_amd/python/profile_summary.py stats <kernel_stats.csv>
   python tope-fft_amd/python/profile_summary.py pmc <fetch.csv> <write.csv> \
          <copy_fetch.csv> <copy_write.csv> <copy_bytes>
+  python tope-fft_amd/python/profile_summary.py counters <kernel-substring> \
+         <counter_collection.csv> [...]
 
 `pmc` scales each kernel's median FETCH_SIZE and WRITE_SIZE (KiB per
 dispatch) by the ratio the same counter shows for a plain copy of
@@ -52,8 +54,25 @@ def pmc(fetch, write, cfetch, cwrite, copy_bytes):
         print(f"traffic_bytes_per_launch {k} {b:.0f}")
 
 
+def counters(pattern, paths):
+    """Median per dispatch of every counter in the given runs, for each
+    kernel whose name contains `pattern` (one row per kernel)."""
+    vals = {}
+    for path in paths:
+        for r in csv.DictReader(open(path)):
+            k = short(r["Kernel_Name"])
+            if pattern in k:
+                vals.setdefault(k, {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+    for k, d in vals.items():
+        print(f"{k}")
+        for c in sorted(d):
+            print(f"    {c:32s} {statistics.median(d[c]):16.0f}   (dispatches: {len(d[c])})")
+
+
 if __name__ == "__main__":
     if sys.argv[1] == "stats":
         stats(sys.argv[2])
+    elif sys.argv[1] == "counters":
+        counters(sys.argv[2], sys.argv[3:])
     else:
         pmc(sys.argv[2], sys.argv[3], sys.argv[4], sys.argv[5], int(sys.argv[6]))
