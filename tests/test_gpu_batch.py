@@ -32,8 +32,9 @@ def run_batch(n, count, x, prec, mode, direction):
         fw.destroy()
 
 
-# powers of two; 7-smooth lengths (mixed-radix pass); others (Bluestein)
-@pytest.mark.parametrize("n", [2, 16, 256, 1024, 4096, 8192, 3, 12, 100, 1000, 4000, 11, 1009, 2048 - 1])
+# powers of two; 7-smooth lengths (mixed-radix pass, four-step of two above
+# 4096); others (Bluestein)
+@pytest.mark.parametrize("n", [2, 16, 256, 1024, 4096, 8192, 3, 12, 100, 1000, 4000, 6000, 11, 1009, 2048 - 1])
 def test_batch_fp64(oracle, n):
     count = 12
     x = batch_input(n, count, 100 + n)
@@ -121,10 +122,11 @@ def test_batch_round_trip_mfma():
     assert rel_l2(back.astype(np.complex128), x.astype(np.complex128)) < 2e-6
 
 
-@pytest.mark.parametrize("n", [12, 100, 1000, 3000, 8192, 16384])
+@pytest.mark.parametrize("n", [12, 100, 1000, 3000, 6000, 8192, 16384])
 def test_batch_complex64_mixed_radix(oracle, n):
-    # complex64 batch of 7-smooth lengths (the mixed-radix pass in FP32) and
-    # of powers of two above 4096 (the four-step in FP32)
+    # complex64 batch of 7-smooth lengths (the mixed-radix pass in FP32, the
+    # mixed four-step above 4096) and of powers of two above 4096 (the
+    # four-step in FP32)
     count = 10
     x = batch_input(n, count, 300 + n).astype(np.complex64)
     x64 = x.astype(np.complex128)

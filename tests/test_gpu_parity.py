@@ -99,6 +99,46 @@ def test_1d_mixed_radix(oracle, n):
     assert rel_l2(topefft.transform(x, (n,), topefft.INVERSE), oracle.dft(x, (n,), topefft.INVERSE)) < TOL
 
 
+@pytest.mark.parametrize("n", [4116, 6000, 10000, 19683])
+def test_1d_mixed_four_step(oracle, n):
+    # 7-smooth lengths above 4096: two mixed-radix passes (n = n1 n2, both at
+    # most 4096, four-step with the W_n^{n1 k2} post-twiddle), not Bluestein
+    x = seeded((n,))
+    fw = topefft.Framework()
+    p = topefft.Plan(fw, (n,), x)
+    names = p.pass_names()
+    fwd = p.exec(topefft.FORWARD)
+    back = p.exec(topefft.INVERSE)
+    p.destroy()
+    fw.destroy()
+    assert names == ["mixed_radix_f64", "mixed_radix_f64"], names
+    assert rel_l2(fwd, oracle.dft(x, (n,), topefft.FORWARD)) < TOL, n
+    assert rel_l2(back, x) < TOL, n
+    assert rel_l2(topefft.transform(x, (n,), topefft.INVERSE), oracle.dft(x, (n,), topefft.INVERSE)) < TOL
+
+
+def test_1d_mixed_four_step_large(oracle):
+    # 10^6 = 1000 x 1000 and 5^9 = 3125 x 625: sampled bins against the
+    # direct DFT, the round trip, and a planted exponential's closed form
+    for n in (1000000, 1953125):
+        x = seeded((n,), salt=3)
+        k0 = 12345
+        x += np.exp(2j * np.pi * ((k0 * np.arange(n)) % n) / n)
+        fw = topefft.Framework()
+        p = topefft.Plan(fw, (n,), x)
+        names = p.pass_names()
+        X = p.exec(topefft.FORWARD)
+        back = p.exec(topefft.INVERSE)
+        p.destroy()
+        fw.destroy()
+        assert names == ["mixed_radix_f64", "mixed_radix_f64"], names
+        rng = np.random.default_rng(n)
+        bins = np.concatenate([[0, 1, k0, n - 1], rng.integers(0, n, 40)])
+        assert rel_l2(X[bins], oracle.dft_bins(x, (n,), bins, topefft.FORWARD)) < TOL, n
+        assert rel_l2(back, x) < TOL, n
+        assert abs(X[k0] - n) < 1e-2 * n
+
+
 @pytest.mark.parametrize("n", [105, 1000, 2187])
 def test_1d_mixed_radix_closed_forms(n):
     # delta -> all ones; exp(2 pi i k0 j / n) -> n at k0 only
@@ -145,7 +185,7 @@ def test_1d_non_pow2_dft(gfw, oracle, n):
         assert rel_l2(got, oracle.dft(x, (n,), d)) < TOL, (n, d)
 
 
-@pytest.mark.parametrize("n", [65, 4097, 10007, 12000, 30000])
+@pytest.mark.parametrize("n", [65, 4097, 10007, 12007, 30011])
 def test_1d_bluestein(oracle, n):
     # non-power-of-two lengths above 64 run Bluestein's chirp-z convolution
     # over power-of-two passes (length M >= 2n - 1, four-step above 4096)
@@ -160,7 +200,7 @@ def test_1d_bluestein(oracle, n):
     assert names[0] == "chirp_pre" and names[-1] == "chirp_post", names
     assert rel_l2(fwd, oracle.dft(x, (n,), topefft.FORWARD)) < TOL, n
     assert rel_l2(back, x) < TOL, n
-    if n <= 12000:
+    if n <= 12007:
         assert rel_l2(topefft.transform(x, (n,), topefft.INVERSE), oracle.dft(x, (n,), topefft.INVERSE)) < TOL
 
 
@@ -343,9 +383,11 @@ def test_error_paths():
 
 
 # the reference exits on these (src/fft2d.c:355-362, src/fft3d.c:522-533):
-# 7-smooth axes above 4096 and other axes above 2048 run Bluestein with an
-# inner FFT of 8192 or 16384 points (four-step)
-LONG_NON_POW2 = [(6000, 4), (4, 5000), (3001, 2), (4097, 3), (2, 4100, 3), (3, 2, 4099)]
+# 7-smooth axes above 4096 run the mixed-radix four-step (two mixed-radix
+# passes, 6000 = 80 x 75, 5000 = 100 x 50, 4900 = 70 x 70), other axes above
+# 2048 Bluestein with an inner FFT of 8192 or 16384 points (four-step)
+LONG_NON_POW2 = [(6000, 4), (4, 5000), (3001, 2), (4097, 3), (2, 4100, 3), (3, 2, 4099), (3, 6000, 2),
+                 (2, 3, 4900), (5000, 2, 3), (7, 4116)]
 
 
 @pytest.mark.parametrize("dims", LONG_NON_POW2, ids=lambda d: "x".join(map(str, d)))
@@ -358,7 +400,10 @@ def test_nd_long_non_pow2(oracle, dims):
     back = p.exec(topefft.INVERSE)
     p.destroy()
     fw.destroy()
-    assert "chirp_pre" in names, names
+    long_smooth = [n for n in dims if n > 4096 and smooth7(n)]
+    short_mixed = [n for n in dims if n <= 4096 and n & (n - 1) and smooth7(n)]
+    assert names.count("mixed_radix_f64") == 2 * len(long_smooth) + len(short_mixed), names
+    assert ("chirp_pre" in names) == any(not smooth7(n) for n in dims), names
     assert rel_l2(fwd, oracle.dft(x, dims, topefft.FORWARD)) < TOL, dims
     assert rel_l2(back, x) < TOL, dims
     assert rel_l2(topefft.transform(x, dims, topefft.INVERSE), oracle.dft(x, dims, topefft.INVERSE)) < TOL
@@ -394,7 +439,8 @@ def test_2d_dif2_schedule_small(oracle, monkeypatch, dims):
 
 
 @pytest.mark.parametrize("dims", [(16, 16), (64, 32), (8, 16, 4), (32, 32, 32), (64, 4, 8),
-                                  (12, 10), (100, 6), (15, 7, 9), (1000, 3), (3000, 2),
+                                  (12, 10), (100, 6), (15, 7, 9), (1000, 3), (3000, 2), (6000, 3), (2, 5000),
+                                  (3, 2, 4900),
                                   (8192, 4), (4, 8192), (16384, 2), (2, 2, 8192), (5, 2000), (7, 1125, 2)],
                          ids=lambda d: "x".join(map(str, d)))
 def test_nd_complex64(oracle, dims):

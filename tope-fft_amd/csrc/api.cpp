@@ -507,6 +507,63 @@ class Planner {
     // one), chirp post-multiply back into the line.  The data stays in
     // buffer *cur.  The filters are transformed here, at plan time, by a
     // one-row FFT_M.
+    // 7-smooth axis longer than 4096: the four-step i = n1 + N1 n2 over two
+    // mixed-radix passes (n = N1 N2, both at most 4096).  Step 1 transforms
+    // n2 (length N2, stride N1 is) for each (line, n1) into the other buffer
+    // in the same layout, multiplying output k2 by W_n^{n1 k2}; step 2
+    // transforms n1 (length N1, stride is) for each (line, k2) and writes
+    // X[k2 + N2 k1] back, so the data ends where it started.
+    int mixed_four_step(long long n, const Geo &g, int *cur, int axis_id)
+    {
+        long long N1 = 0;
+        if (!mixed4_split(n, &N1)) return TOPE_ERROR_INVALID_SIZE;
+        const long long N2 = n / N1;
+        long long S = 1;
+        while (S * S < n) ++S;
+        void *r1 = nullptr, *r2 = nullptr, *lo = nullptr, *hi = nullptr;
+        hipError_t err = power_table(N2, 1, N2, &r1);
+        if (err == hipSuccess) err = power_table(N1, 1, N1, &r2);
+        if (err == hipSuccess) err = power_table(n, 1, S, &lo);
+        if (err == hipSuccess) err = power_table(n, S, (n + S - 1) / S, &hi);
+        if (err != hipSuccess) return map_err(err);
+        auto mixed_pass = [&](long long len, const Fold &f, long long in_is, long long out_is, int bi, int bo,
+                              const void *roots, bool post) {
+            Pass ps;
+            ps.kind = PASS_MIXED;
+            ps.mx = mixed_plan(len, f.in.O, f.in.os, f.in.C, f.in.cs, in_is);
+            ps.mx.oos = f.out.os;
+            ps.mx.ocs = f.out.cs;
+            ps.mx.ois = out_is;
+            ps.mx.roots = roots;
+            if (post) {
+                ps.mx.pt_lo = lo;
+                ps.mx.pt_hi = hi;
+                ps.mx.pt_n = n;
+                ps.mx.pt_o = f.new_in_o ? 1 : 0;
+                ps.mx.pt_c = f.new_in_c ? 1 : 0;
+                ps.mx.pt_split = (int)S;
+            }
+            ps.grid = mixed_grid(ps.mx);
+            ps.in_buf = bi;
+            ps.out_buf = bo;
+            ps.reps = f.reps;
+            ps.rep_in = f.rep_in;
+            ps.rep_out = f.rep_out;
+            ps.name = P->prec == PREC_F64 ? "mixed_radix_f64" : "mixed_radix_f32";
+            P->passes.push_back(ps);
+        };
+        // step 1: lines (old o, old c, n1)
+        mixed_pass(N2, fold(g, N1, g.is, g.is), N1 * g.is, N1 * g.is, *cur, *cur ^ 1, r1, true);
+        // step 2: lines (old o, old c, k2)
+        mixed_pass(N1, fold(g, N2, N1 * g.is, g.is), g.is, N2 * g.is, *cur ^ 1, *cur, r2, false);
+        if (axis_id >= 0 && !P->axis_tw[axis_id].table) {
+            P->axis_tw[axis_id].n = (int)N2;
+            P->axis_tw[axis_id].e = 0;
+            P->axis_tw[axis_id].table = r1;
+        }
+        return TOPE_SUCCESS;
+    }
+
     // Axis of 7-smooth length n <= 4096 that is not a power of two: one
     // in-place mixed-radix pass (kernels_mixed.hip), twiddles and radix-3/5/7
     // constants from a table of W_n^i.
@@ -783,8 +840,10 @@ int launch_pass(PlanImpl *P, size_t i, hipStream_t s, int dir)
     } else if (ps.kind == PASS_MIXED) {
         MixedArgs m = ps.mx;
         m.scale = ps.last ? sc : 1.0;
-        hipError_t err = launch_mixed(P->prec, ib, ob, m, sign, s);
-        if (err != hipSuccess) return map_err(err);
+        for (long long r = 0; r < ps.reps; ++r) {
+            hipError_t err = launch_mixed(P->prec, ib + r * ps.rep_in * eb, ob + r * ps.rep_out * eb, m, sign, s);
+            if (err != hipSuccess) return map_err(err);
+        }
     } else if (ps.kind == PASS_CHIRP) {
         // 1/M undoes the unnormalised inner inverse FFT; the plan's 1/N of
         // INVERSE rides on its last pass as for the axis passes
@@ -867,7 +926,9 @@ int build_plan(struct topeFFT *f, int shape, long long nx, long long ny, long lo
     // complex64 axes: powers of two, or 7-smooth up to 4096 (mixed-radix
     // pass) on the vector ALUs; FP64 also takes Bluestein for the others
     // (a batch transforms only its rows: ny is the count)
-    auto f32_axis = [](long long n) { return is_pow2(n) || mixed_supported(n); };
+    long long split_ = 0;
+    auto mixed4 = [&](long long n) { return mixed4_split(n, &split_); };
+    auto f32_axis = [&](long long n) { return is_pow2(n) || mixed_supported(n) || mixed4(n); };
     const bool batch = shape == SHAPE_BATCH;
     const bool f32_ok = f32_axis(nx) && (batch || (f32_axis(ny) && f32_axis(nz)));
     const bool all_pow2 = is_pow2(nx) && (batch || (is_pow2(ny) && is_pow2(nz)));
@@ -902,7 +963,7 @@ int build_plan(struct topeFFT *f, int shape, long long nx, long long ny, long lo
         const long long dims3[3] = {nx, ny, nz};
         for (int ax = 0; ax < (shape == SHAPE_BATCH ? 1 : 3); ++ax) {
             const long long n = dims3[ax];
-            if (n == 1 || is_pow2(n) || mixed_supported(n) || (shape == 1 && n <= kChirpMin)) continue;
+            if (n == 1 || is_pow2(n) || mixed_supported(n) || mixed4(n) || (shape == 1 && n <= kChirpMin)) continue;
             // M above 4096 runs its FFTs as the four-step, through the
             // data buffer (consumed by the chirp pre-multiply by then)
             const long long M = Planner::chirp_len(n);
@@ -936,6 +997,7 @@ int build_plan(struct topeFFT *f, int shape, long long nx, long long ny, long lo
     auto any_axis = [&](Planner &pl, long long n, const Geo &g, int *cur, int id) {
         if (n > 1 && !is_pow2(n)) {
             if (mixed_supported(n)) return pl.mixed_axis(n, g, *cur, id);
+            if (mixed4(n)) return pl.mixed_four_step(n, g, cur, id);
             return pl.bluestein_axis(n, g, cur);
         }
         return pl.axis(n, g, cur, id);
@@ -976,7 +1038,7 @@ int build_plan(struct topeFFT *f, int shape, long long nx, long long ny, long lo
         P->axis_tw[0].table = tw;
     } else if (shape == SHAPE_BATCH) {
         rc = any_axis(pl, nx, Geo{1, 0, ny, nx, 1}, &cur, 0);
-    } else if (nonpow2 && !(shape == 1 && !chirpz && !mixed_supported(nx))) {
+    } else if (nonpow2 && !(shape == 1 && !chirpz && !mixed_supported(nx) && !mixed4(nx))) {
         rc = any_axis(pl, nx, Geo{1, 0, ny * nz, nx, 1}, &cur, 0);
         if (!rc && shape >= 2) rc = any_axis(pl, ny, Geo{nz, nx * ny, nx, 1, nx}, &cur, 1);
         if (!rc && shape >= 3) rc = any_axis(pl, nz, Geo{1, 0, nx * ny, 1, nx * ny}, &cur, 2);

@@ -95,8 +95,19 @@ struct MixedArgs {
     int radix[16];
     const void *roots;  // W_n^i, i < n, in the pass's precision (forward sign)
     double scale;
+    // output geometry (mixed_plan sets it equal to the input's)
+    long long oos, ocs, ois;
+    // optional four-step post-twiddle: output k of line (o, c) times
+    // W_pt_n^{(o pt_o + c pt_c) k}, the exponent below pt_n <= 2^24 without
+    // reduction; W_pt_n^a = pt_lo[a mod pt_split] * pt_hi[a / pt_split]
+    const void *pt_lo, *pt_hi;
+    long long pt_n, pt_o, pt_c;
+    int pt_split;
 };
 bool mixed_supported(long long n);
+// 7-smooth n above MIXED_MAX_N that splits as n = n1 n2 with both factors
+// at most MIXED_MAX_N (mixed-radix four-step); *n1 >= n2 is the split used.
+bool mixed4_split(long long n, long long *n1);
 MixedArgs mixed_plan(long long n, long long O, long long os, long long C, long long cs, long long is);
 unsigned mixed_grid(const MixedArgs &a);
 hipError_t launch_mixed(int prec, const void *in, void *out, const MixedArgs &a, int sign, hipStream_t st);

@@ -157,6 +157,11 @@ struct RowOut {
     Div64 dC;
     Div32 dtb;
     R scale;
+    long long oos, ocs, ois;  // output geometry
+    // four-step post-twiddle (pt_n != 0)
+    const cpx<R> *pt_lo, *pt_hi;
+    long long pt_n, pt_o, pt_c;
+    Div32 dsplit;
 };
 
 // One stage of compile-time radix r over the block's lines, LDS to LDS,
@@ -205,13 +210,20 @@ __device__ __forceinline__ void mixed_stage(const cpx<R> *src, cpx<R> *dst, int 
             if (l < go.lines) {
                 long long c;
                 const long long o = divmod64(l, go.dC, c);
-                cpx<R> *row = go.out + o * go.os + c * go.cs;
+                cpx<R> *row = go.out + o * go.oos + c * go.ocs;
                 const int k0 = jn * Ns * r + kk;
+                // four-step post-twiddle base: the exponent (base k) < pt_n
+                const int base = go.pt_n ? (int)(o * go.pt_o + c * go.pt_c) : 0;
 #pragma unroll
                 for (int m = 0; m < r; ++m) {
-                    const cpx<R> v{x[m].x * go.scale, x[m].y * go.scale};
-                    if constexpr (CF) row[(long long)(k0 + m * Ns) * go.is] = v;
-                    else row[k0 + m * Ns] = v;
+                    cpx<R> v{x[m].x * go.scale, x[m].y * go.scale};
+                    if (go.pt_n) {
+                        int lo;
+                        const int hi = divmod(base * (k0 + m * Ns), go.dsplit, lo);
+                        const cpx<R> w = cmul(go.pt_lo[lo], go.pt_hi[hi]);
+                        v = SIGN < 0 ? cmul(v, w) : cmulc(v, w);
+                    }
+                    row[(long long)(k0 + m * Ns) * go.ois] = v;
                 }
             }
         } else {
@@ -251,6 +263,7 @@ __global__ void __launch_bounds__(NT) mixed_kernel(const cpx<R> *in, cpx<R> *out
     const int n = a.n, tb = a.tb;
     const int span_elems = tb * n;
     const bool cfast = a.is != 1;  // strided axis: neighbouring lanes take neighbouring lines
+    const bool cfast_out = a.ois != 1;  // the same for the last stage's stores
     // strided lines share 128-byte segments with the neighbouring blocks'
     // lines: keep neighbours on one XCD (one L2)
     const long long l0 = xcd_block(cfast ? 1 : 0) * tb;
@@ -272,7 +285,25 @@ __global__ void __launch_bounds__(NT) mixed_kernel(const cpx<R> *in, cpx<R> *out
     // lines itself (no initial LDS fill and barrier) and the last stage
     // stores straight to the output (no final LDS round trip); on strided
     // axes those two stages put neighbouring lines on neighbouring lanes
-    const RowOut<R> go{in, out, l0, a.lines, a.os, a.cs, a.is, dC, dtb, (R)a.scale};
+    const RowOut<R> go{in,
+                       out,
+                       l0,
+                       a.lines,
+                       a.os,
+                       a.cs,
+                       a.is,
+                       dC,
+                       dtb,
+                       (R)a.scale,
+                       a.oos,
+                       a.ocs,
+                       a.ois,
+                       static_cast<const cpx<R> *>(a.pt_lo),
+                       static_cast<const cpx<R> *>(a.pt_hi),
+                       a.pt_n,
+                       a.pt_o,
+                       a.pt_c,
+                       mkdiv(a.pt_split > 0 ? a.pt_split : 1)};
     const bool direct_first = a.nst >= 2;
     cpx<R> *src = lds, *dst = lds + SLOTS;
     int Ns = 1;
@@ -297,7 +328,7 @@ __global__ void __launch_bounds__(NT) mixed_kernel(const cpx<R> *in, cpx<R> *out
     for (; s < a.nst; ++s) {
         const int r = a.radix[s];
         if (s == a.nst - 1) {
-            if (cfast) mixed_stage_r<R, SIGN, NT, true, false, true>(r, src, dst, n, tb, Ns, roots, go);
+            if (cfast_out) mixed_stage_r<R, SIGN, NT, true, false, true>(r, src, dst, n, tb, Ns, roots, go);
             else mixed_stage_r<R, SIGN, NT, true>(r, src, dst, n, tb, Ns, roots, go);
             return;
         }
@@ -347,7 +378,26 @@ MixedArgs mixed_plan(long long n, long long O, long long os, long long C, long l
     if (tb > a.lines) tb = a.lines;
     a.tb = (int)tb;
     a.scale = 1.0;
+    a.oos = os;
+    a.ocs = cs;
+    a.ois = is;
     return a;
+}
+
+bool mixed4_split(long long n, long long *n1)
+{
+    if (n <= MIXED_MAX_N || n > (long long)MIXED_MAX_N * MIXED_MAX_N) return false;
+    long long m = n;
+    for (long long p : {2, 3, 5, 7})
+        while (m % p == 0) m /= p;
+    if (m != 1) return false;
+    // the smallest divisor d >= sqrt(n) with d, n / d <= MIXED_MAX_N
+    for (long long d = 1; d <= MIXED_MAX_N; ++d) {
+        if (n % d || d * d < n || n / d > MIXED_MAX_N) continue;
+        *n1 = d;
+        return true;
+    }
+    return false;
 }
 
 unsigned mixed_grid(const MixedArgs &a) { return (unsigned)((a.lines + a.tb - 1) / a.tb); }
