@@ -63,11 +63,14 @@ def measure(fw, plan, steps, warmup):
 def kernel_symbol(variant_name, sign=-1):
     """axis_double_n4096_e16_tb4_col_split -> the kernel's template name
     prefix as rocprofv3 prints it (forward, no post-twiddle); the
-    even/odd-split column tiles (..._dit2) are dit2_kernel."""
+    even/odd-split column tiles (..._dit2) are dit2_kernel, the three-step
+    column tile (..._ct3) ct3_kernel."""
     parts = variant_name.split("_")
     prec, n, e, tb = parts[1], parts[2][1:], parts[3][1:], parts[4][2:]
     col = "true" if parts[5] == "col" else "false"
     split = "true" if "split" in parts[6:] else "false"
+    if "ct3" in parts[6:]:
+        return f"tope::ct3_kernel<{prec}, {sign}, false"
     if "dit2" in parts[6:]:
         return f"tope::dit2_kernel<{prec}, {n}, {e}, {tb}, {split}, {sign}, false"
     return f"tope::axis_kernel<{prec}, {n}, {e}, {tb}, {col}, {split}, {sign}, false"
@@ -309,6 +312,26 @@ def vendor_leg(steps):
     return [json.loads(line) for line in r.stdout.splitlines() if line.startswith("{")]
 
 
+def pattern_yardsticks(reps=20):
+    """The access patterns of the two 2D 4096^2 passes as plain in-place
+    copies of the same 256 MiB array (tope-fft_amd/lib/copy_calib: rows of
+    4096 elements per 256-thread block; 4-column tiles per 1024-thread block
+    laid out per XCD), timed with HIP events in this run: the floors the X
+    and Y passes are set against."""
+    import subprocess
+    exe = os.path.join(REPO, "tope-fft_amd", "lib", "copy_calib")
+    if not os.path.exists(exe):
+        return {"skipped": "copy_calib not built"}
+    out = {}
+    for mode in ("rows", "cols"):
+        r = subprocess.run([exe, "256", str(reps), mode], capture_output=True, text=True, timeout=120)
+        if r.returncode != 0:
+            return {"error": (r.stderr or r.stdout)[-300:]}
+        us = float(r.stdout.split(":")[1].split()[0])
+        out[mode] = {"us": us, "gbs": round(2 * 256 * 2**20 / (us * 1e-6) / 1e9, 1)}
+    return out
+
+
 def cpu_library_fft(x, dims, cores):
     """A tuned CPU FFT beside the restated reference: scipy.fft (pocketfft)
     fft2 on the same input with `cores` worker threads, median of 5."""
@@ -353,7 +376,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="uneven batch: skip the hipGraph-replay methods (for kernel-trace profiling)")
     ap.add_argument("--legs", default="all",
-                    help="comma list of mp2d,1d,3d256,mp,many,3d512,r2c,vendor (default all; 2d is always run)")
+                    help="comma list of mp2d,1d,3d256,mp,many,3d512,r2c,vendor,yard (default all; 2d is always run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -382,7 +405,7 @@ def main():
     # at most 20 steps each so the default run stays short
     side_steps = max(5, min(20, args.steps // 2))
     legs = set(args.legs.split(",")) if args.legs != "all" else {"mp2d", "1d", "3d256", "mp", "many", "3d512", "r2c",
-                                                                  "vendor"}
+                                                                  "vendor", "yard"}
     extra = {}
     if "mp2d" in legs:
         extra["mixed_precision_2d_4096"] = mp_2d_leg(fw, x2, t2, args.steps, args.warmup)
@@ -412,6 +435,9 @@ def main():
         extra["3d_512cubed"] = large_3d_leg(fw, 4, 1)
     if "vendor" in legs:
         extra["vendor_rocfft"] = vendor_leg(side_steps)
+    yard = pattern_yardsticks() if "yard" in legs else None
+    if yard is not None:
+        extra["pattern_yardsticks"] = yard
 
     cpu = None
     if not args.no_cpu:
@@ -424,6 +450,10 @@ def main():
         extra["cpu_library_fft"] = cpu_library_fft(x2, dims2, cores)
 
     flops2 = 5.0 * n2 * np.log2(n2)
+    # pass i of the 2D plan is the row (0) or column (1) pattern
+    pattern_us = None
+    if isinstance(yard, dict) and "rows" in yard and len(names2) == 2:
+        pattern_us = [yard["rows"]["us"], yard["cols"]["us"]]
     line = {
         "metric": "GB/s & transforms/s, FP64 vs mixed-prec MFMA at rel-L2<=1e-6, 2D 4096^2 C2C",
         "value": round(value, 2),
@@ -450,7 +480,11 @@ def main():
                      "traffic_source": f"{TRAFFIC_FILE}: rocprofv3 FETCH_SIZE and WRITE_SIZE of this kernel, "
                                        "each scaled by what it reads for a plain copy (a separate profiled run)",
                      "per_launch_algorithmic_bytes": dom_bytes,
-                     "avg_launch_ms": round(float(pass2[dom]), 5)},
+                     "avg_launch_ms": round(float(pass2[dom]), 5),
+                     # the same kernel against an in-place copy in its own
+                     # access pattern (column tiles), measured in this run
+                     "frac_of_pattern_copy": (round(pattern_us[dom] / (float(pass2[dom]) * 1e3), 4)
+                                              if pattern_us else None)},
         "cpu_baseline": cpu,
     }
     p2.destroy()

@@ -21,10 +21,11 @@
  *     directions (the reference's DFT kernel ignores `dir`,
  *     src/kernels1D.cl:549-573): 7-smooth lengths (factors 2, 3, 5, 7) up
  *     to 4096 by a mixed-radix pass, other lengths by a GPU DFT up to 64
- *     and Bluestein's chirp-z convolution above; 2D/3D axes that are not
- *     powers of two (7-smooth up to 4096 by the mixed-radix pass, every
- *     other length up to 2^23 by Bluestein) run the same way where the
- *     reference exits (src/fft2d.c:355-362).
+ *     and Bluestein's chirp-z convolution above; 7-smooth lengths above
+ *     4096 that split into two factors of at most 4096 run a four-step of
+ *     two mixed-radix passes.  2D/3D axes that are not powers of two run the
+ *     same way (Bluestein for lengths up to 2^23) where the reference exits
+ *     (src/fft2d.c:355-362).
  *   - the n == 2 1D inverse divides both real and imaginary parts by 2
  *     (src/fft1d.c:23-26 divides only the real parts).
  *   - tope*Destroy frees the plan; the framework (stream) is released when
@@ -204,8 +205,9 @@ const char *topePlanPassName(void *plan_impl, int pass);
  *                         transform scaled by a power of two into fp16 range
  * MFMA modes take n = 256, 512, 1024, 2048 or 4096.  Twiddles and DFT-matrix constants are
  * generated in FP64 on the GPU and rounded once; INVERSE scales by 1/n.
- * Both precisions take powers of two and 7-smooth n up to 4096 (mixed-radix
- * pass, VALU mode); FP64 plans also take other n up to 2^23 (Bluestein). */
+ * Both precisions take powers of two and 7-smooth n (mixed-radix pass up to
+ * 4096, a four-step of two such passes above; VALU mode); FP64 plans also
+ * take other n up to 2^23 (Bluestein). */
 #define TOPE_PREC_F64 0
 #define TOPE_PREC_F32 1
 #define TOPE_MODE_VALU 0
@@ -236,7 +238,7 @@ void topeBatchDestroy(struct topeFFT *f, struct topePlanBatch *t);
 /* 2D/3D plans in a chosen precision: TOPE_PREC_F64 (complex128, host
  * double*, identical to tope2DPlanInit/tope3DPlanInit) or TOPE_PREC_F32
  * (complex64 storage, host float*, FP32 butterflies with FP64-generated
- * twiddles; axes powers of two or 7-smooth up to 4096).  Exec/ExecDevice/Destroy as for the reference plans; the Ex
+ * twiddles; axes powers of two or 7-smooth).  Exec/ExecDevice/Destroy as for the reference plans; the Ex
  * Exec variants take the host buffer as void*. */
 void tope2DPlanInitEx(struct topeFFT *f, struct topePlan2D *t, int x, int y, int type, int precision,
                       const void *host);

@@ -158,6 +158,29 @@ def test_many_full_20000(oracle):
     check_against_oracle(oracle, sizes, x, out, 1, per_class=1)
 
 
+def test_many_full_20000_complex64(oracle):
+    # the same batch in complex64 (FP32 storage and arithmetic): two
+    # transforms of every length class against the naive DFT of the
+    # complex64 input, and the whole batch against the FP64 plan
+    sizes = workloads.uneven_sizes(workloads.SEED_MANY_SIZES, 20000)
+    x = workloads.complex_uniform(workloads.SEED_MANY_DATA, int(sizes.sum())).astype(np.complex64)
+    x64 = x.astype(np.complex128)
+    fw = topefft.Framework()
+    p = topefft.ManyPlan(fw, sizes, x, topefft.PREC_F32)
+    p.set_method(topefft.MANY_PERSISTENT_GRAPH)
+    out = p.exec(topefft.FORWARD).astype(np.complex128)
+    p.destroy()
+    p64 = topefft.ManyPlan(fw, sizes, x64)
+    ref = p64.exec(topefft.FORWARD)
+    p64.destroy()
+    fw.destroy()
+    seen = check_against_oracle(oracle, sizes, x64, out, 1, per_class=2, tol=1e-6)
+    assert sorted(seen) == sorted(set(int(n) for n in sizes))
+    err = rel_l2(out, ref)
+    print(f"20000-transform complex64 batch vs FP64 rel-L2 {err:.3e}")
+    assert err < 1e-6
+
+
 def test_many_rejects_bad_sizes():
     fw = topefft.Framework()
     with pytest.raises(topefft.TopeError) as e:

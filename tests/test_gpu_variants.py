@@ -78,6 +78,7 @@ def test_every_preferred_variant_is_compiled():
     # exist, or the preference silently falls through
     names = {v[0] for v in topefft.axis_variants()}
     for want in ("axis_double_n4096_e16_tb1_row_split", "axis_double_n4096_e16_tb4_col_split",
+                 "axis_double_n4096_e16_tb4_col_ct3",
                  "axis_double_n2048_e16_tb4_col_split", "axis_double_n256_e16_tb32_col_split",
                  "axis_double_n512_e16_tb16_col_split", "axis_double_n256_e16_tb4_row",
                  "axis_float_n512_e16_tb32_col_split", "axis_float_n256_e16_tb32_col_split",

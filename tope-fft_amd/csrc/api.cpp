@@ -25,6 +25,7 @@
 #include "internal.hpp"
 #include "mfma_kernel.hpp"
 #include "mfma_tile.hpp"
+#include "axis_ct3.hpp"
 
 using namespace tope;
 
@@ -93,6 +94,7 @@ struct PlanImpl {
     size_t bytes = 0;      // the transform's data (host transfers)
     size_t buf_bytes = 0;  // device arrays when larger than bytes (Bluestein's padded length)
     void *buf[2] = {nullptr, nullptr};
+    void *ct3_tw = nullptr;  // twiddle table of the three-step column tiles
     int final_buf = 0;
     std::vector<Pass> passes;
     std::vector<void *> allocs;
@@ -190,9 +192,13 @@ const AxisVariant *find_variant_named(const char *name)
 struct Preferred {
     const char *name;
     long long min_blocks;
+    bool strided_only = false;  // only when input and output elements are both strided
 };
 constexpr long long kMinBlocks = 256;  // one block per CU at least, when the shape allows
 const Preferred kPreferred[] = {
+    // the three-step tile's lane groups read rows 16 apart: strided lines only
+    // (2D 4096^2 Y pass 103.8-104.2 us against 110-112 us for the one-shot tile)
+    {"axis_double_n4096_e16_tb4_col_ct3", kMinBlocks, true},
     {"axis_double_n4096_e16_tb1_row_split", kMinBlocks},
     {"axis_double_n4096_e16_tb4_col_split", kMinBlocks},
     {"axis_double_n2048_e16_tb4_col_split", kMinBlocks},
@@ -210,32 +216,48 @@ const Preferred kPreferred[] = {
 // (wider tiles give longer contiguous segments per row), preferring the
 // full-complex exchange at equal width.  TOPEFFT_VARIANT_<n>_<row|col>=<name>
 // overrides (used by tuning sweeps).
-// plain_only: one-shot variants only (no even/odd split, which needs an
-// extra table), for callers that provide no tables of their own.
+// plain_only: one-shot variants only (no even/odd split or three-step
+// tile, which need tables of their own), for callers that provide no tables.
+// strided_io: the line elements are strided in the input and the output.
 const AxisVariant *select_variant(int prec, int n, bool cfast, long long lines, bool dif2 = false,
-                                  long long outer = 1, bool plain_only = false)
+                                  long long outer = 1, bool plain_only = false, bool strided_io = false)
 {
     char key[64];
     std::snprintf(key, sizeof key, "TOPEFFT_VARIANT_%d_%s%s", n, cfast ? "col" : "row", dif2 ? "_dif2" : "");
     if (const char *ov = std::getenv(key)) {
         const AxisVariant *v = find_variant_named(ov);
-        if (v && v->prec == prec && v->n == n && v->dif2 == dif2 && !(plain_only && v->dit2)) return v;
+        if (v && v->prec == prec && v->n == n && v->dif2 == dif2 && !(plain_only && (v->dit2 || v->ct3))) return v;
     }
     // blocks a tile width gives for this shape (small shapes: keep the CUs busy)
     auto blocks = [&](const AxisVariant *v) { return outer * ((lines + v->tb - 1) / v->tb); };
     for (const Preferred &pr : kPreferred) {
         const AxisVariant *v = find_variant_named(pr.name);
-        if (v && plain_only && v->dit2) continue;
+        if (v && plain_only && (v->dit2 || v->ct3)) continue;
+        if (pr.strided_only && !strided_io) continue;
         if (v && v->prec == prec && v->n == n && v->cfast == cfast && v->dif2 == dif2 && v->tb <= lines &&
             blocks(v) >= pr.min_blocks)
             return v;
     }
     int cnt = 0;
     const AxisVariant *vs = axis_variants(&cnt);
-    // the even/odd-split tiles compete only through the preference table
-    auto matches = [&](const AxisVariant *v) {
-        return v->prec == prec && v->n == n && v->cfast == cfast && v->dif2 == dif2 && !v->dit2;
+    // the even/odd-split tiles compete only through the preference table;
+    // single-column tiles of a strided axis (16-byte row segments) only
+    // when every wider tile leaves more than half of the CUs without a
+    // block (1024 x 512: 512-point columns in 1-column tiles 8.1 us against
+    // 6.4 us in 8-column tiles, 128 blocks; 1D 2^16: 256-point columns 3.5
+    // against 4.2 us in 4-column tiles, 64 blocks)
+    auto base_match = [&](const AxisVariant *v) {
+        return v->prec == prec && v->n == n && v->cfast == cfast && v->dif2 == dif2 && !v->dit2 && !v->ct3;
     };
+    long long most_wide = 0;
+    {
+        int c0 = 0;
+        const AxisVariant *v0 = axis_variants(&c0);
+        for (int i = 0; i < c0; ++i)
+            if (base_match(&v0[i]) && v0[i].tb > 1) most_wide = std::max(most_wide, blocks(&v0[i]));
+    }
+    const bool allow_single = !cfast || most_wide < kMinBlocks / 2;
+    auto matches = [&](const AxisVariant *v) { return base_match(v) && (allow_single || v->tb > 1); };
     // Small shapes: when no tile gives every CU a block, take the tile that
     // gives the most blocks (full-complex exchange first); otherwise only
     // tiles that still give every CU a block compete.
@@ -365,17 +387,34 @@ class Planner {
     {
         const bool dif2 = pair != 0;
         const bool cfast = !(gi.is == 1 && go.is == 1);
-        const AxisVariant *v = select_variant(P->prec, n, cfast, dif2 ? 2 * gi.C : gi.C, dif2, gi.O * reps);
+        const AxisVariant *v =
+            select_variant(P->prec, n, cfast, dif2 ? 2 * gi.C : gi.C, dif2, gi.O * reps, false, gi.is != 1 && go.is != 1);
         if (!v) return TOPE_ERROR_NO_KERNEL;
         // even/odd-split tiles run the stages of an n/2-point transform and
         // combine the halves with W_n^k, k < n/2 (passed as dif_tw)
         const int tn = v->dit2 ? v->n / 2 : v->n;
-        HostSched h = host_sched(tn, v->e);
-        if (h.twsize != v->twsize) return TOPE_ERROR_NO_KERNEL;  // host/device layouts disagree
         void *tw = nullptr;
-        // (kind stays PASS_AXIS)
-        hipError_t err = stage_table(tn, v->e, &tw);
-        if (err != hipSuccess) return map_err(err);
+        hipError_t err = hipSuccess;
+        if (v->ct3) {
+            // W_256^j then W_4096^i (j, i < 128), one table per plan
+            if (!P->ct3_tw) {
+                void *d = nullptr;
+                err = hipMalloc(&d, (size_t)CT3_TW * P->elem);
+                if (err == hipSuccess) err = gen_power_table(d, P->prec, 256, 1, 128, st);
+                if (err == hipSuccess)
+                    err = gen_power_table(static_cast<char *>(d) + 128 * P->elem, P->prec, 4096, 1, 128, st);
+                if (d) P->allocs.push_back(d);
+                if (err != hipSuccess) return map_err(err);
+                P->ct3_tw = d;
+            }
+            tw = P->ct3_tw;
+        } else {
+            HostSched h = host_sched(tn, v->e);
+            if (h.twsize != v->twsize) return TOPE_ERROR_NO_KERNEL;  // host/device layouts disagree
+            // (kind stays PASS_AXIS)
+            err = stage_table(tn, v->e, &tw);
+            if (err != hipSuccess) return map_err(err);
+        }
         if (v->dit2) {
             void *wk = nullptr;
             err = power_table(v->n, 1, v->n / 2, &wk);
@@ -383,9 +422,16 @@ class Planner {
             dif_tw = wk;
         }
         if (axis >= 0 && !P->axis_tw[axis].table) {
+            // the axis's exported table is the stage table of (n, e) for
+            // every variant (the three-step tile keeps its own beside it)
+            void *stw = tw;
+            if (v->ct3) {
+                err = stage_table(tn, v->e, &stw);
+                if (err != hipSuccess) return map_err(err);
+            }
             P->axis_tw[axis].n = tn;
             P->axis_tw[axis].e = v->e;
-            P->axis_tw[axis].table = tw;
+            P->axis_tw[axis].table = stw;
         }
         Pass ps;
         ps.var = v;

@@ -1,0 +1,178 @@
+// Strided 4096-point pass with one of its two exchanges inside each wave.
+//
+// The one-shot column tile (axis_kernel.hpp) runs the 4096-point transform
+// as three radix-16 Stockham stages with two block-wide exchanges, each
+// behind workgroup barriers, so every wave waits for the slowest wave's
+// loads before the second stage.  Here the same tile (TB = 4 columns, 1024
+// threads, 16 elements per thread) runs a three-step Cooley-Tukey
+// decomposition, n = 256 n2 + 16 n1 + n0, k = k0 + 16 k1 + 256 k2:
+//   A: for each (n1, n0): DFT-16 over n2                 -> A[k0; n1, n0]
+//   B: times W_256^{n1 k0}, DFT-16 over n1               -> B[k0, k1; n0]
+//   C: times W_4096^{n0 (k0 + 16 k1)}, DFT-16 over n0    -> X[k0 + 16 k1 + 256 k2]
+// with thread (wave w, lane group g, column t) holding n0 = w, n1 = g in
+// step A.  The exchange between A and B swaps the lane-group index with the
+// register index inside one wave (the wave's own LDS rows, no workgroup
+// barrier); only the exchange between B and C crosses waves.  A wave whose
+// loads arrive early runs steps A and B while others still wait for theirs.
+// Both exchanges go through one LDS region per wave (16 rows of 68 values
+// of one component): the in-wave transpose reads and writes the wave's own
+// region, and the cross-wave exchange has each wave write its values into
+// its own region (rows of 64) and read the other waves' regions.  So no
+// workgroup barrier precedes the cross-wave writes, and the steps A and B of
+// each wave run as soon as its own loads arrive.  All accesses are
+// bank-conflict free (rows of 68 doubles for the transpose; contiguous
+// 64-value rows for the cross-wave exchange).
+//
+// Replaces, like the axis kernel, the reference's per-stage strided passes
+// (src/fft2d.c:74-134 with src/kernels2D.cl:15-288; src/fft3d.c:80-214 with
+// src/kernels3D.cl:15-298).
+#pragma once
+#include "axis_kernel.hpp"
+
+namespace tope {
+
+// Twiddle table of the pass (a.tw): W_256^j, j < 128, then W_4096^i, i < 128
+// (steps B and C use exponents m g, m w < 128, m = 1, 2, 4, 8).
+constexpr int CT3_TW = 128 + 128;
+
+template <typename R>
+struct Ct3Lds {
+    static constexpr int WR = 16 * 68;                     // values of one component per wave region
+    static constexpr int XB = 16 * WR * (int)sizeof(R);    // 16 wave regions
+    static constexpr int BYTES = XB + CT3_TW * (int)sizeof(cpx<R>);  // + the twiddle table
+};
+
+// Orders a wave's LDS writes before its reads of other lanes' values (and
+// those reads before the wave's next writes): LDS operations of one wave
+// execute in order; this keeps the compiler from moving them across.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename R, int SIGN, bool POST, bool BIG>
+__global__ void __launch_bounds__(1024) ct3_kernel(const cpx<R> *in, cpx<R> *out, const AxisArgs<R> a)
+{
+    constexpr int TB = 4;
+    constexpr int CB = (int)sizeof(cpx<R>);
+    using L = Ct3Lds<R>;
+    __shared__ __align__(16) unsigned char lds[L::BYTES];
+    R *xl = reinterpret_cast<R *>(lds);
+    cpx<R> *w256 = reinterpret_cast<cpx<R> *>(lds + L::XB);
+    const cpx<R> *w4k = w256 + 128;
+
+    const int tid = threadIdx.x;
+    const int t = tid & 3, g = (tid >> 2) & 15, w = tid >> 6;
+    // step C's twiddle rows (read after the first workgroup barrier)
+    if (tid < CT3_TW) w256[tid] = a.tw[tid];
+
+    const long long blk = xcd_block(a.xcd);
+    const long long o = blk / a.tiles;
+    int tile = (int)(blk - o * a.tiles) + a.rot;
+    if (tile >= a.tiles) tile -= a.tiles;
+    const int c0 = tile * TB;
+    const int c = c0 + t;
+    const bool valid = c < a.C;
+
+    // step A inputs: rows 256 e + 16 g + w of column c
+    cpx<R> v[16];
+    if constexpr (!BIG) {
+        const long long ibase = o * a.in_os + (long long)c0 * a.in_cs;
+        const __amdgpu_buffer_rsrc_t rin = make_rsrc(in + ibase, (a.in_elems - ibase) * CB);
+        const int voff = (int)((t * a.in_cs + (long long)(16 * g + w) * a.in_is) * CB);
+        const int sstep = (int)(256LL * a.in_is * CB);
+        if (valid) {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) v[e] = buf_load<R>(rin, voff, e * sstep);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) v[e] = cpx<R>{0, 0};
+        }
+    } else {
+        if (valid) {
+            const cpx<R> *src = in + o * a.in_os + (long long)c * a.in_cs;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) v[e] = src[(long long)(256 * e + 16 * g + w) * a.in_is];
+        } else {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) v[e] = cpx<R>{0, 0};
+        }
+    }
+
+    dft16<SIGN>(v);  // register k0
+
+    R *mine = xl + w * L::WR;  // this wave's region
+    // in-wave exchange: register k0 -> lane group k0, lane group n1 -> register n1
+    {
+        const int wb = 4 * g + t, rb = g * 68 + t;
+#pragma unroll
+        for (int comp = 0; comp < 2; ++comp) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) mine[k * 68 + wb] = comp ? v[k].y : v[k].x;
+            wave_lds_sync();
+#pragma unroll
+            for (int r = 0; r < 16; ++r) (comp ? v[r].y : v[r].x) = mine[rb + 4 * r];
+            wave_lds_sync();
+        }
+    }
+
+    // step B: register n1 times W_256^{n1 k0} (k0 = g), DFT over n1 -> register k1
+    // (the W_256 rows come from the global table: no barrier before this step)
+    {
+        cpx<R> tw[16];
+#pragma unroll
+        for (int m = 1; m < 16; m <<= 1) tw[m] = a.tw[m * g];
+        apply_twiddles<SIGN>(v, tw);
+    }
+    dft16<SIGN>(v);
+
+    // cross-wave exchange: register k1 -> wave k1, wave n0 -> register n0;
+    // wave w writes row k of its own region, wave k reads row k of every region
+    {
+        const int lane = 4 * g + t;
+#pragma unroll
+        for (int comp = 0; comp < 2; ++comp) {
+            if (comp) __syncthreads();  // every wave has read the real parts
+#pragma unroll
+            for (int k = 0; k < 16; ++k) mine[k * 64 + lane] = comp ? v[k].y : v[k].x;
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < 16; ++r) (comp ? v[r].y : v[r].x) = xl[r * L::WR + w * 64 + lane];
+        }
+    }
+
+    // step C: register n0 times W_4096^{n0 b}, b = g + 16 w; DFT over n0 -> register k2
+    {
+        cpx<R> tw[16];
+#pragma unroll
+        for (int m = 1; m < 16; m <<= 1) tw[m] = cmul(w4k[m * g], w256[m * w]);
+        apply_twiddles<SIGN>(v, tw);
+    }
+    dft16<SIGN>(v);
+
+    if (!valid) return;
+    // output rows k = g + 16 w + 256 e
+    if constexpr (POST)
+        post_twiddle<SIGN>(v, o * a.pt_o + (long long)c * a.pt_c, g + 16 * w, 256, a.pt_lo, a.pt_hi, a.pt_n,
+                           a.pt_shift);
+    if (a.scale != R(1)) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) v[e] = scale(v[e], a.scale);
+    }
+    if constexpr (!BIG) {
+        const long long obase = o * a.out_os + (long long)c0 * a.out_cs;
+        const __amdgpu_buffer_rsrc_t rout = make_rsrc(out + obase, (a.out_elems - obase) * CB);
+        const int voff = (int)((t * a.out_cs + (long long)(g + 16 * w) * a.out_is) * CB);
+        const int sstep = (int)(256LL * a.out_is * CB);
+#pragma unroll
+        for (int e = 0; e < 16; ++e) buf_store<R>(v[e], rout, voff, e * sstep);
+    } else {
+        cpx<R> *dst = out + o * a.out_os + (long long)c * a.out_cs;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) dst[(long long)(g + 16 * w + 256 * e) * a.out_is] = v[e];
+    }
+}
+
+}  // namespace tope

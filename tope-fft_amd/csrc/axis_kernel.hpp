@@ -185,6 +185,31 @@ __device__ __forceinline__ void buf_store(cpx<R> v, __amdgpu_buffer_rsrc_t r, in
         __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(tope_u32x2, v), r, voff, soff, 0);
 }
 
+// Four-step post-twiddle of one thread's outputs k = p + P e (e < E) of a
+// line with twiddle base b: v[e] *= W_{pt_n}^{b k}.  W^{b (p + P e)} is
+// W^{b p} (W^{b P})^e: a fresh table value every RS outputs, products of
+// the step W^{b P} in between (FP64: one chain of E - 1 products, relative
+// error ~E ulp; FP32: fresh every 4 outputs).  W_{pt_n}^a is
+// pt_lo[a & mask] * pt_hi[a >> pt_shift]; b k < 2^63.
+template <int SIGN, typename R, int E>
+__device__ __forceinline__ void post_twiddle(cpx<R> (&v)[E], long long b, long long p, long long P,
+                                             const cpx<R> *pt_lo, const cpx<R> *pt_hi, long long pt_n, int pt_shift)
+{
+    constexpr int RS = sizeof(R) == 8 ? E : 4;
+    const long long nm = pt_n - 1, lm = (1LL << pt_shift) - 1;
+    auto look = [&](long long ex) {
+        ex &= nm;
+        return cmul(pt_lo[ex & lm], pt_hi[ex >> pt_shift]);
+    };
+    const cpx<R> s = look(b * P);
+    cpx<R> w = look(b * p);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        if (e) w = (e % RS == 0) ? look(b * (p + P * e)) : cmul(w, s);
+        v[e] = twmul<SIGN>(v[e], w);
+    }
+}
+
 // Twiddles W^{k*m}, m = 1..r-1, of one butterfly: only the power-of-two
 // exponents m = 1, 2, 4, 8 are read; every other one is the product of two
 // earlier entries (m = 2^a + b, b < 2^a), at most three products deep, and
@@ -463,17 +488,8 @@ __device__ __forceinline__ void axis_block(const cpx<R> *in, cpx<R> *out, const 
     }
 
     if (!valid) return;
-    if constexpr (POST) {
-        const long long base = o * a.pt_o + (long long)c * a.pt_c;
-        const long long mask = (1LL << a.pt_shift) - 1;
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            const long long k = p + P * e;
-            const long long ex = (base * k) & (a.pt_n - 1);
-            const cpx<R> w = cmul(a.pt_lo[ex & mask], a.pt_hi[ex >> a.pt_shift]);
-            v[e] = twmul<SIGN>(v[e], w);
-        }
-    }
+    if constexpr (POST)
+        post_twiddle<SIGN>(v, o * a.pt_o + (long long)c * a.pt_c, p, P, a.pt_lo, a.pt_hi, a.pt_n, a.pt_shift);
     if (a.scale != R(1)) {  // uniform: only the last pass of an inverse scales
 #pragma unroll
         for (int e = 0; e < E; ++e) v[e] = scale(v[e], a.scale);

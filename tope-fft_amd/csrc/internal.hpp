@@ -26,8 +26,9 @@ struct AxisVariant {
     bool dif2;    // row pairs + radix-2 DIF step of the next axis
     bool dit2;    // column tiles split into even/odd halves (axis_dit2.hpp):
                   // stage table of (n/2, e), W_n^k table (k < n/2) in dif_tw
+    bool ct3;     // three-step column tile (axis_ct3.hpp): its own twiddle table
     int threads;  // block size
-    int twsize;   // stage twiddle entries for (n, e), (n/2, e) when dit2
+    int twsize;   // stage twiddle entries for (n, e), (n/2, e) when dit2, CT3_TW when ct3
     int lds_bytes;
     AxisLaunchFn fn;
     const char *name;

@@ -7,6 +7,7 @@
 // instruction covers 64/TB rows of TB contiguous elements (TB*16 bytes for
 // FP64); the split-exchange column variants trade two extra barriers per
 // stage for twice as many columns per block.
+#include "axis_ct3.hpp"
 #include "axis_dit2.hpp"
 #include "internal.hpp"
 
@@ -60,10 +61,28 @@ static void launch_dit2_t(const void *in, void *out, const void *args, unsigned 
     }
 }
 
+// Three-step column tiles (ct3_kernel): 4096 points, 4 columns, 1024 threads.
+template <typename R, bool BIG>
+static void launch_ct3_t(const void *in, void *out, const void *args, unsigned grid, hipStream_t s, int sign)
+{
+    const AxisArgs<R> &a = *static_cast<const AxisArgs<R> *>(args);
+    const cpx<R> *i = static_cast<const cpx<R> *>(in);
+    cpx<R> *o = static_cast<cpx<R> *>(out);
+    if (a.pt_n) {
+        if (sign < 0) hipLaunchKernelGGL((ct3_kernel<R, -1, true, BIG>), dim3(grid), dim3(1024), 0, s, i, o, a);
+        else hipLaunchKernelGGL((ct3_kernel<R, 1, true, BIG>), dim3(grid), dim3(1024), 0, s, i, o, a);
+    } else {
+        if (sign < 0) hipLaunchKernelGGL((ct3_kernel<R, -1, false, BIG>), dim3(grid), dim3(1024), 0, s, i, o, a);
+        else hipLaunchKernelGGL((ct3_kernel<R, 1, false, BIG>), dim3(grid), dim3(1024), 0, s, i, o, a);
+    }
+}
+
 #ifdef TOPE_AXIS_BIG
+#define TOPE_LAUNCH_CT3(R) &launch_ct3_t<R, true>
 #define TOPE_LAUNCH(R, N, E, TB, CF, SP, DF) &launch_axis_t<R, N, E, TB, CF, SP, true, DF>
 #define TOPE_LAUNCH_DIT2(R, N, E, TB, SP) &launch_dit2_t<R, N, E, TB, SP, true>
 #else
+#define TOPE_LAUNCH_CT3(R) &launch_ct3_t<R, false>
 #define TOPE_LAUNCH(R, N, E, TB, CF, SP, DF) &launch_axis_t<R, N, E, TB, CF, SP, false, DF>
 #define TOPE_LAUNCH_DIT2(R, N, E, TB, SP) &launch_dit2_t<R, N, E, TB, SP, false>
 #endif
@@ -80,15 +99,20 @@ constexpr int prec_of() { return sizeof(R) == 8 ? PREC_F64 : PREC_F32; }
 #define TOPE_STR2(x) #x
 #define TOPE_STR(x) TOPE_STR2(x)
 #define VARD(R, N, E, TB, CF, SP, DF)                                                                        \
-    AxisVariant{prec_of<R>(), N, E, TB, CF, SP, DF, false, TB * (N / E), Sched<N, E>::TWSIZE,               \
+    AxisVariant{prec_of<R>(), N, E, TB, CF, SP, DF, false, false, TB * (N / E), Sched<N, E>::TWSIZE,        \
                 Exchange<R, N, E, TB, CF, SP>::BYTES, TOPE_LAUNCH(R, N, E, TB, CF, SP, DF),                \
                 "axis_" TOPE_STR(R) "_n" TOPE_STR(N) "_e" TOPE_STR(E) "_tb" TOPE_STR(TB) TOPE_LAYOUT_##CF \
                     TOPE_SPLIT_##SP TOPE_DIF2_##DF}
 #define VAR(R, N, E, TB, CF, SP) VARD(R, N, E, TB, CF, SP, false)
 #define VARH(R, N, E, TB, SP)                                                                                \
-    AxisVariant{prec_of<R>(), N, E, TB, true, SP, false, true, TB * (N / 2 / E), Sched<N / 2, E>::TWSIZE,  \
+    AxisVariant{prec_of<R>(), N, E, TB, true, SP, false, true, false, TB * (N / 2 / E), Sched<N / 2, E>::TWSIZE, \
                 Dit2Lds<R, N, E, TB, SP>::BYTES, TOPE_LAUNCH_DIT2(R, N, E, TB, SP),                        \
                 "axis_" TOPE_STR(R) "_n" TOPE_STR(N) "_e" TOPE_STR(E) "_tb" TOPE_STR(TB) "_col" TOPE_SPLIT_##SP "_dit2"}
+
+// Three-step column tile (4096 points, 4 columns, 16 elements per thread).
+#define VARC(R)                                                                                               \
+    AxisVariant{prec_of<R>(), 4096, 16, 4, true, true, false, false, true, 1024, CT3_TW, Ct3Lds<R>::BYTES, \
+                TOPE_LAUNCH_CT3(R), "axis_" TOPE_STR(R) "_n4096_e16_tb4_col_ct3"}
 
 // Even/odd-split column tiles for the long strided axes.
 #define DIT2COLS(R) VARH(R, 4096, 8, 4, true), VARH(R, 4096, 16, 4, true), VARH(R, 4096, 16, 4, false), \
@@ -132,7 +156,8 @@ constexpr int prec_of() { return sizeof(R) == 8 ? PREC_F64 : PREC_F32; }
 
 static const AxisVariant kVariants[] = {ROWS(double), COLS(double), DIF2ROWS(double), ROWS(float), COLS(float),
                                         DIT2COLS(double), VARH(float, 4096, 16, 8, true),
-                                        VARH(float, 4096, 16, 8, false), VAR(float, 4096, 32, 8, true, true)};
+                                        VARH(float, 4096, 16, 8, false), VAR(float, 4096, 32, 8, true, true),
+                                        VARC(double)};
 
 // The same list is compiled twice: normally (32-bit buffer offsets) and with
 // TOPE_AXIS_BIG (64-bit pointers) for blocks spanning more than 2^31 bytes.
