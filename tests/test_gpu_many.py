@@ -170,6 +170,8 @@ def test_many_full_20000_complex64(oracle):
     p.set_method(topefft.MANY_PERSISTENT_GRAPH)
     out = p.exec(topefft.FORWARD).astype(np.complex128)
     p.destroy()
+    fw.destroy()
+    fw = topefft.Framework()  # the last plan's destroy released the first one
     p64 = topefft.ManyPlan(fw, sizes, x64)
     ref = p64.exec(topefft.FORWARD)
     p64.destroy()
