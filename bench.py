@@ -24,6 +24,7 @@ import argparse
 import json
 import os
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -176,7 +177,8 @@ def uneven_batch_leg(fw, steps, warmup, graphs=True):
     for prec, name, xs, sb in ((topefft.PREC_F64, "fp64", x, 16),
                                (topefft.PREC_F32, "complex64", x.astype(np.complex64), 8)):
         p = topefft.ManyPlan(fw, sizes, xs, prec)
-        p.tune(None, reps=3)
+        wisdom = os.path.join(tempfile.mkdtemp(prefix="topefft_"), "wisdom.txt")
+        p.tune(wisdom, reps=3)
         radix, grid = p.schedule()
         entry = {"schedule_radix_by_log2n": {str(L): radix[L] for L in range(6, 15)}, "grid": grid}
         methods = (topefft.MANY_PERSISTENT, topefft.MANY_PERSISTENT_GRAPH, topefft.MANY_PER_TRANSFORM,
@@ -200,9 +202,41 @@ def uneven_batch_leg(fw, steps, warmup, graphs=True):
                                                     "gbs": round(gbs(2 * sb * total, t), 1),
                                                     "transforms_per_s": round(len(sizes) / t, 1),
                                                     "device_ms_hip_events": round(dev_ms, 4)}
+        entry["per_class_alone"] = per_class_times(fw, sizes, xs, prec, sb, wisdom, max(5, steps))
         p.destroy()
         res[name] = entry
     return res
+
+
+def per_class_times(fw, sizes, xs, prec, sb, wisdom, reps):
+    """Each length class of the uneven batch alone (its transforms packed in
+    their own plan, the tuned schedule from `wisdom`, persistent method):
+    device ms per exec from HIP events and the HBM rate.  The 1024-thread
+    classes (n >= 8192) own a CU per block; the others run four blocks per CU."""
+    import topefft
+    offs = np.concatenate([[0], np.cumsum(sizes)[:-1]])
+    out = {}
+    old = os.environ.get("TOPEFFT_WISDOM")
+    os.environ["TOPEFFT_WISDOM"] = wisdom
+    try:
+        for L in range(6, 15):
+            sel = np.flatnonzero(sizes == (1 << L))
+            if not len(sel):
+                continue
+            xc = np.concatenate([xs[offs[i]:offs[i] + sizes[i]] for i in sel])
+            pc = topefft.ManyPlan(fw, sizes[sel], xc, prec)
+            pc.set_method(topefft.MANY_PERSISTENT)
+            pc.exec_device(topefft.FORWARD)
+            ms = pc.time_exec(reps)
+            pc.destroy()
+            out[str(L)] = {"transforms": int(len(sel)), "device_ms_hip_events": round(ms, 4),
+                           "gbs": round(gbs(2 * sb * float(sizes[sel].sum()), ms * 1e-3), 1)}
+    finally:
+        if old is None:
+            os.environ.pop("TOPEFFT_WISDOM", None)
+        else:
+            os.environ["TOPEFFT_WISDOM"] = old
+    return out
 
 
 def large_3d_leg(fw, steps, warmup):

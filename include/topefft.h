@@ -24,8 +24,9 @@
  *     and Bluestein's chirp-z convolution above; 7-smooth lengths above
  *     4096 that split into two factors of at most 4096 run a four-step of
  *     two mixed-radix passes.  2D/3D axes that are not powers of two run the
- *     same way (Bluestein for lengths up to 2^23) where the reference exits
- *     (src/fft2d.c:355-362).
+ *     same way (Bluestein for lengths up to 2^29) where the reference exits
+ *     (src/fft2d.c:355-362).  Power-of-two axes run one pass up to 4096,
+ *     the four-step (two passes) up to 2^24 and three passes up to 2^30.
  *   - the n == 2 1D inverse divides both real and imaginary parts by 2
  *     (src/fft1d.c:23-26 divides only the real parts).
  *   - tope*Destroy frees the plan; the framework (stream) is released when
@@ -207,7 +208,7 @@ const char *topePlanPassName(void *plan_impl, int pass);
  * generated in FP64 on the GPU and rounded once; INVERSE scales by 1/n.
  * Both precisions take powers of two and 7-smooth n (mixed-radix pass up to
  * 4096, a four-step of two such passes above; VALU mode); FP64 plans also
- * take other n up to 2^23 (Bluestein). */
+ * take other n up to 2^29 (Bluestein). */
 #define TOPE_PREC_F64 0
 #define TOPE_PREC_F32 1
 #define TOPE_MODE_VALU 0
@@ -277,7 +278,8 @@ void topePlanExecPtr(struct topeFFT *f, void *plan_impl, void *data, int dir);
 /* ---- real-input transforms (no reference counterpart; the reference
  * declares R2C but runs C2C) ---------------------------------------------
  * x (even) by y by z (1 for unused axes), FP64, x fastest; x/2, y and z
- * each a power of two up to 2^24 or 7-smooth (factors 2, 3, 5, 7) up to 4096
+ * each a power of two up to 2^30 or 7-smooth (factors 2, 3, 5, 7) up to 4096,
+ * or above when it splits into two 7-smooth factors of at most 4096
  * (others: TOPE_ERROR_UNSUPPORTED, odd x: TOPE_ERROR_INVALID_SIZE).
  * PlanInit uploads the real array (x*y*z doubles).
  * topeRealExec FORWARD (R2C) transforms the device real array into the half

@@ -5,6 +5,8 @@ Bars: FP64 rel-L2 <= 1e-12.  FP32 storage: plain FP32 arithmetic (VALU and
 MFMA fp32 operands) and the 3-part bf16 split <= 1e-6; the 2-part bf16 split
 (dropped terms ~2^-16) <= 1e-4.  The measured errors are printed.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -152,11 +154,13 @@ def test_batch_rejects_unsupported():
     with pytest.raises(topefft.TopeError) as e:
         topefft.BatchPlan(fw, 100, 2, np.zeros(200, np.complex64), topefft.PREC_F32, topefft.MODE_MFMA_FP16X3)
     assert e.value.code == topefft.TOPE_ERROR_UNSUPPORTED
-    # Bluestein rows of M = 2^k >= 2n - 1 up to 2^24 points
-    n = (1 << 23) + 3
-    with pytest.raises(topefft.TopeError) as e:
-        topefft.BatchPlan(fw, n, 1, np.zeros(n, complex), topefft.PREC_F64, topefft.MODE_VALU)
-    assert e.value.code == topefft.TOPE_ERROR_INVALID_SIZE
+    # Bluestein rows of M = 2^k >= 2n - 1 up to 2^30 points: n = 2^29 + 1 is
+    # refused at planning, before the input (a few bytes here) is read
+    t = topefft.TopePlanBatch()
+    tiny = np.zeros(4, complex)
+    fw.lib.topeBatchPlanInit(C.byref(fw.f), C.byref(t), (1 << 29) + 1, 1, topefft.PREC_F64, topefft.MODE_VALU,
+                             tiny.ctypes.data_as(C.c_void_p))
+    assert fw.f.error == topefft.TOPE_ERROR_INVALID_SIZE
     fw.destroy()
 
 

@@ -1,0 +1,114 @@
+"""Axes longer than 4096^2 points against the oracle.
+
+Power-of-two lengths above 2^24 run three passes (n = A B C, each factor
+at most 4096: a post-twiddled pass over the outer index, the four-step of
+the inner A B points); non-power-of-two lengths whose Bluestein row
+(M >= 2n - 1) is longer than 2^24 run their inner FFTs that way.  The
+transforms are too long for the full naive DFT, so each is checked at
+sampled bins against the oracle's direct DFT (FP64: rel-L2 <= 1e-12;
+complex64 storage <= 1e-6), by Parseval and by the round trip on the
+device; the pass lists are asserted.  The 2^27-point FP64 array (2 GiB)
+takes the 64-bit addressing variants of the passes.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import topefft
+import workloads
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def seeded(n, salt):
+    return workloads.complex_uniform(0x10A6 + salt * 7919 + n % 100003, n)
+
+
+def sample_bins(n, seed, k=10):
+    rng = np.random.default_rng(seed)
+    return np.concatenate([[0, 1, n - 1, n // 2, n // 3], rng.integers(0, n, k)])
+
+
+def check_1d(oracle, n, x, salt, tol=TOL, nbins=10):
+    fw = topefft.Framework()
+    p = topefft.Plan(fw, (n,), x)
+    names = p.pass_names()
+    X = p.exec(topefft.FORWARD)
+    back = p.exec(topefft.INVERSE)
+    p.destroy()
+    bins = sample_bins(n, salt, nbins)
+    assert rel_l2(X[bins], oracle.dft_bins(x, (n, 1, 1), bins, topefft.FORWARD)) < tol
+    ex, eX = np.vdot(x, x).real, np.vdot(X, X).real
+    assert abs(eX - n * ex) / (n * ex) < tol
+    assert rel_l2(back, x) < tol
+    return names
+
+
+def test_1d_pow2_three_pass(oracle):
+    n = 1 << 25  # 512 x 256 x 256
+    names = check_1d(oracle, n, seeded(n, 1), 1)
+    assert len(names) == 3 and "_n512_" in names[0] and "_n256_" in names[1] and "_n256_" in names[2], names
+
+
+def test_1d_pow2_three_pass_64bit_addressing(oracle):
+    # 2^27 points (2 GiB FP64): 512 x 512 x 512, spans beyond 2^31 bytes
+    n = 1 << 27
+    names = check_1d(oracle, n, seeded(n, 2), 2, nbins=4)
+    assert len(names) == 3 and all("_n512_" in nm for nm in names), names
+
+
+def test_batch_three_pass(oracle):
+    n, count = 1 << 25, 2
+    x = seeded(n * count, 3)
+    fw = topefft.Framework()
+    p = topefft.BatchPlan(fw, n, count, x, topefft.PREC_F64, topefft.MODE_VALU)
+    X = p.exec(topefft.FORWARD)
+    back = p.exec(topefft.INVERSE)
+    p.destroy()
+    fw.destroy()
+    for r in range(count):
+        seg = slice(r * n, (r + 1) * n)
+        bins = sample_bins(n, 30 + r, 6)
+        assert rel_l2(X[seg][bins], oracle.dft_bins(x[seg], (n, 1, 1), bins, topefft.FORWARD)) < TOL, r
+    assert rel_l2(back, x) < TOL
+
+
+def test_batch_three_pass_complex64(oracle):
+    n = 1 << 25
+    x = seeded(n, 4).astype(np.complex64)
+    fw = topefft.Framework()
+    p = topefft.BatchPlan(fw, n, 1, x, topefft.PREC_F32, topefft.MODE_VALU)
+    X = p.exec(topefft.FORWARD).astype(np.complex128)
+    back = p.exec(topefft.INVERSE).astype(np.complex128)
+    p.destroy()
+    fw.destroy()
+    x64 = x.astype(np.complex128)
+    bins = sample_bins(n, 40, 10)
+    err = rel_l2(X[bins], oracle.dft_bins(x64, (n, 1, 1), bins, topefft.FORWARD))
+    print(f"complex64 2^25 rel-L2 at sampled bins {err:.3e}")
+    assert err < 1e-6
+    assert rel_l2(back, x64) < 1e-6
+
+
+def test_1d_bluestein_three_pass(oracle):
+    # 16777219 = 1549 x 10831: Bluestein row M = 2^26 (inner FFTs in three passes)
+    n = (1 << 24) + 3
+    names = check_1d(oracle, n, seeded(n, 5), 5, nbins=6)
+    assert "chirp_pre" in names and "chirp_post" in names, names
+
+
+def test_refuses_beyond_the_three_pass_limit():
+    # a Bluestein row of 2^31 points (n = 2^29 + 1) is refused at planning,
+    # before anything is allocated or the input is read (the buffer passed
+    # is a few bytes)
+    fw = topefft.Framework()
+    t = topefft.TopePlan1D()
+    tiny = np.zeros(4)
+    fw.lib.tope1DPlanInit(C.byref(fw.f), C.byref(t), (1 << 29) + 1, topefft.C2C,
+                          tiny.ctypes.data_as(C.POINTER(C.c_double)))
+    assert fw.f.error == topefft.TOPE_ERROR_INVALID_SIZE
+    fw.destroy()

@@ -8,6 +8,8 @@ Parseval).  Calls follow the reference's sequence: topeFFTInit,
 tope*PlanInit (uploads), tope*Exec (transforms on the device, downloads),
 tope*Destroy.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -368,12 +370,15 @@ def test_profile_counters_and_passes():
 
 def test_error_paths():
     gfw = topefft.Framework()
-    # a 2D/3D axis longer than 2^23 that is not a power of two: its
-    # Bluestein length would exceed 2^24
-    n = (1 << 23) + 1
-    with pytest.raises(topefft.TopeError) as e:
-        topefft.Plan(gfw, (n, 1), np.zeros(n, complex))
-    assert e.value.code == topefft.TOPE_ERROR_INVALID_SIZE, n
+    # a 2D/3D axis longer than 2^29 that is not a power of two: its
+    # Bluestein length would exceed 2^30; refused at planning, before the
+    # input (a few bytes here) is read
+    t = topefft.TopePlan2D()
+    tiny = np.zeros(4)
+    gfw.lib.tope2DPlanInit(C.byref(gfw.f), C.byref(t), (1 << 29) + 1, 1, topefft.C2C,
+                           tiny.ctypes.data_as(C.POINTER(C.c_double)))
+    assert gfw.f.error == topefft.TOPE_ERROR_INVALID_SIZE
+    gfw.f.error = 0
     p = topefft.Plan(gfw, (16,), np.zeros(16, complex))
     with pytest.raises(topefft.TopeError) as e:
         p.exec(5)

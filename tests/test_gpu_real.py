@@ -2,7 +2,8 @@
 oracle's full complex transform (sliced to kx <= nx/2), C2R by round trip and
 against the oracle's inverse, rows above 8192 and columns above 4096
 through the four-step (full comparison up to 32768 points per axis, sampled
-bins at 2^22), 7-smooth axes through the mixed-radix pass, complex64
+bins at 2^22), rows of 2^25 complex points in three passes (sampled bins),
+7-smooth axes through the mixed-radix pass (and its four-step above 4096), complex64
 storage (1e-6), closed forms, and the size limits.  Bar:
 rel-L2 <= 1e-12, as for the complex FP64 path."""
 import numpy as np
@@ -17,7 +18,10 @@ pytestmark = pytest.mark.gpu
 # half-spectrum step on its own
 SHAPES = [(2,), (4,), (16,), (256,), (4096,), (8192,), (16384,), (32768,), (16384, 4), (16384, 2, 2),
           (16, 8192), (4, 2, 8192), (200,), (24, 10), (30, 6, 10), (2000, 3), (8, 4), (64, 32), (256, 256), (2, 8),
-          (16, 8, 4), (32, 32, 32), (2, 4, 8), (128, 2, 16)]
+          (16, 8, 4), (32, 32, 32), (2, 4, 8), (128, 2, 16),
+          # 7-smooth axes above 4096: the mixed-radix four-step (x/2 = 6000 =
+          # 80 x 75 and 5000 = 100 x 50 on the rows, y = 6000 on the columns)
+          (12000,), (10000, 3), (16, 6000), (6, 2, 4900)]
 
 
 def real_input(dims, seed):
@@ -129,8 +133,29 @@ def test_r2c_large_four_step_bins(oracle):
     assert rel_l2(back, x) < 1e-12
 
 
+def test_r2c_three_pass_bins(oracle):
+    # x = 2^26: the 2^25-point complex row transform in three passes
+    # (512 x 256 x 256), the half-spectrum step on its own; sampled bins
+    # against the oracle's direct DFT and the C2R round trip
+    n = 1 << 26
+    x = real_input((n,), 78)
+    fw = topefft.Framework()
+    p = topefft.RealPlan(fw, (n,), x)
+    names = p.pass_names()
+    X = p.forward()
+    back = p.inverse()
+    p.destroy()
+    fw.destroy()
+    assert len(names) == 4 and names[-1] == "real_half_spectrum", names
+    rng = np.random.default_rng(10)
+    bins = np.concatenate([[0, 1, n // 2, n // 2 - 1, n // 4], rng.integers(0, n // 2 + 1, 6)])
+    want = oracle.dft_bins(x.astype(np.complex128), (n,), bins, topefft.FORWARD)
+    assert rel_l2(X[bins], want) < 1e-12
+    assert rel_l2(back, x) < 1e-12
+
+
 C64_SHAPES = [(256,), (4096,), (8192,), (64, 32), (256, 256), (16, 8, 4), (2, 8), (200,), (24, 10), (30, 6, 10),
-              (16384,), (32768,), (16, 8192)]
+              (16384,), (32768,), (16, 8192), (12000,), (16, 6000)]
 
 
 @pytest.mark.parametrize("dims", C64_SHAPES, ids=lambda d: "x".join(map(str, d)))

@@ -33,6 +33,7 @@ namespace {
 
 constexpr int kMaxFused = 4096;          // longest axis one pass holds on chip
 constexpr long long kMaxAxis = 1LL << 24;  // four-step limit (4096 x 4096)
+constexpr long long kMaxLen = 1LL << 30;   // three-pass limit of a power-of-two axis here (16 GiB FP64)
 constexpr long long kChirpMin = 64;        // longer non-power-of-two 1D lengths use Bluestein
 
 struct EventPair {
@@ -298,32 +299,42 @@ struct Geo {  // lines (o < O, c < C) of length n, element stride is
     long long O, os, C, cs, is;
 };
 
-// Line enumeration for a four-step step: the axis' own lines (o, c) plus one
-// new index (count, input stride, output stride).  Two of the three go into
-// the kernel's (O, C) slots, C taking the smallest output stride so strided
-// tiles stay contiguous; a third non-trivial index becomes repeated launches.
+// Line enumeration for a four-step step: the axis' own lines (o, c) plus
+// the new indices of the step (count, input stride, output stride; `fresh`
+// marks the one the step's post-twiddle multiplies by).  Two indices go
+// into the kernel's (O, C) slots, C taking the smallest output stride so
+// strided tiles stay contiguous; a third non-trivial index (never the fresh
+// one) becomes repeated launches.  More than three is not expressible.
 struct Fold {
     Geo in{1, 0, 1, 0, 0}, out{1, 0, 1, 0, 0};
     long long reps = 1, rep_in = 0, rep_out = 0;
     bool new_in_o = false, new_in_c = false;
+    bool ok = true;
 };
 
-Fold fold(const Geo &g, long long cnt, long long s_in, long long s_out)
+struct FoldIdx {
+    long long count, si, so;
+    bool fresh;
+};
+
+Fold fold_lines(const Geo &g, std::vector<FoldIdx> extra)
 {
-    struct Idx {
-        long long count, si, so;
-        bool fresh;
-    };
-    std::vector<Idx> v;
+    std::vector<FoldIdx> v;
     if (g.O > 1) v.push_back({g.O, g.os, g.os, false});
     if (g.C > 1) v.push_back({g.C, g.cs, g.cs, false});
-    if (cnt > 1) v.push_back({cnt, s_in, s_out, true});
+    for (const FoldIdx &x : extra)
+        if (x.count > 1) v.push_back(x);
     Fold r;
-    if (v.size() == 3) {  // the axis' outer index becomes repeated launches
-        r.reps = v[0].count;
-        r.rep_in = v[0].si;
-        r.rep_out = v[0].so;
-        v.erase(v.begin());
+    if (v.size() > 3) {
+        r.ok = false;
+        return r;
+    }
+    if (v.size() == 3) {  // a non-fresh index becomes repeated launches
+        size_t k = v[0].fresh ? 1 : 0;
+        r.reps = v[k].count;
+        r.rep_in = v[k].si;
+        r.rep_out = v[k].so;
+        v.erase(v.begin() + (long)k);
     }
     if (v.size() == 2 && v[0].so < v[1].so) std::swap(v[0], v[1]);
     if (v.size() == 2) {
@@ -340,6 +351,11 @@ Fold fold(const Geo &g, long long cnt, long long s_in, long long s_out)
         r.new_in_c = v[0].fresh;
     }
     return r;
+}
+
+Fold fold(const Geo &g, long long cnt, long long s_in, long long s_out)
+{
+    return fold_lines(g, {{cnt, s_in, s_out, true}});
 }
 
 class Planner {
@@ -514,7 +530,8 @@ class Planner {
     {
         if (n == 1) return TOPE_SUCCESS;
         if (n <= kMaxFused) return add_pass((int)n, g, g, *cur, *cur, 1, 0, 0, nullptr, nullptr, 0, 0, 0, 0, axis_id);
-        if (n > kMaxAxis) return TOPE_ERROR_INVALID_SIZE;
+        if (n > kMaxLen) return TOPE_ERROR_INVALID_SIZE;
+        if (n > kMaxAxis) return three_step(n, g, cur, axis_id);
         // four-step: i = n1 + N1*n2.  Step 1 transforms n2 (length N2, stride
         // N1*is) per (line, n1), multiplying output k2 by W_n^{n1*k2}, into
         // the other buffer; step 2 transforms n1 (length N1, stride is) per
@@ -543,6 +560,56 @@ class Planner {
         s2i.is = g.is;
         s2o.is = N2 * g.is;
         return add_pass((int)N1, s2i, s2o, *cur ^ 1, *cur, f2.reps, f2.rep_in, f2.rep_out);
+    }
+
+    // Power-of-two axis longer than 4096^2: n = A B C (each at most 4096),
+    // i = a + A b + A B c, k = k_c + C k_b + C B k_a.  Three passes:
+    //   1. DFT over c (stride A B is) for each (line, a + A b), output k_c
+    //      times W_n^{(a + A b) k_c}, into the other buffer, same layout;
+    //   2. in place there: DFT over b (stride A is) for each (line, a, k_c),
+    //      output k_b times W_{AB}^{a k_b};
+    //   3. DFT over a (stride is) for each (line, k_b, k_c), written back as
+    //      X[k_c + C k_b + C B k_a], so the data ends where it started.
+    // Steps 2 and 3 are the four-step of the inner A B-point transforms.
+    // The axis' own lines may use one index (O or C) at most.
+    int three_step(long long n, const Geo &g, int *cur, int axis_id)
+    {
+        const int L = ilog2ll(n);
+        const int cb = (L + 2) / 3, bb = (L - cb + 1) / 2, ab = L - cb - bb;
+        const long long A = 1LL << ab, B = 1LL << bb, C = 1LL << cb, AB = A * B;
+        if (C > kMaxFused || B > kMaxFused || A > kMaxFused) return TOPE_ERROR_INVALID_SIZE;
+        const long long is = g.is;
+        const Fold f1 = fold_lines(g, {{AB, is, is, true}});
+        const Fold f2 = fold_lines(g, {{C, AB * is, AB * is, false}, {A, is, is, true}});
+        const Fold f3 = fold_lines(g, {{B, A * is, C * is, false}, {C, AB * is, is, false}});
+        if (!f1.ok || !f2.ok || !f3.ok) return TOPE_ERROR_INVALID_SIZE;
+        // W_n^x = lo[x & (2^h - 1)] hi[x >> h]; the same for W_AB
+        auto split_tables = [&](long long m, void **lo, void **hi, int *h) {
+            const int lm = ilog2ll(m);
+            *h = (lm + 1) / 2;
+            hipError_t err = power_table(m, 1, 1LL << *h, lo);
+            if (err == hipSuccess) err = power_table(m, 1LL << *h, m >> *h, hi);
+            return err;
+        };
+        void *lo1 = nullptr, *hi1 = nullptr, *lo2 = nullptr, *hi2 = nullptr;
+        int h1 = 0, h2 = 0;
+        hipError_t err = split_tables(n, &lo1, &hi1, &h1);
+        if (err == hipSuccess) err = split_tables(AB, &lo2, &hi2, &h2);
+        if (err != hipSuccess) return map_err(err);
+        Geo i1 = f1.in, o1 = f1.out;
+        i1.is = o1.is = AB * is;
+        int rc = add_pass((int)C, i1, o1, *cur, *cur ^ 1, f1.reps, f1.rep_in, f1.rep_out, lo1, hi1,
+                          f1.new_in_o ? 1 : 0, f1.new_in_c ? 1 : 0, n, h1, axis_id);
+        if (rc) return rc;
+        Geo i2 = f2.in, o2 = f2.out;
+        i2.is = o2.is = A * is;
+        rc = add_pass((int)B, i2, o2, *cur ^ 1, *cur ^ 1, f2.reps, f2.rep_in, f2.rep_out, lo2, hi2,
+                      f2.new_in_o ? 1 : 0, f2.new_in_c ? 1 : 0, AB, h2);
+        if (rc) return rc;
+        Geo i3 = f3.in, o3 = f3.out;
+        i3.is = is;
+        o3.is = C * B * is;
+        return add_pass((int)A, i3, o3, *cur ^ 1, *cur, f3.reps, f3.rep_in, f3.rep_out);
     }
 
     // Non-power-of-two axis of length n (lines g) by Bluestein's identity
@@ -1009,11 +1076,15 @@ int build_plan(struct topeFFT *f, int shape, long long nx, long long ny, long lo
         const long long dims3[3] = {nx, ny, nz};
         for (int ax = 0; ax < (shape == SHAPE_BATCH ? 1 : 3); ++ax) {
             const long long n = dims3[ax];
+            if (is_pow2(n) && n > kMaxLen) {  // refused before anything is allocated or read
+                delete P;
+                return TOPE_ERROR_INVALID_SIZE;
+            }
             if (n == 1 || is_pow2(n) || mixed_supported(n) || mixed4(n) || (shape == 1 && n <= kChirpMin)) continue;
             // M above 4096 runs its FFTs as the four-step, through the
             // data buffer (consumed by the chirp pre-multiply by then)
             const long long M = Planner::chirp_len(n);
-            if (M > kMaxAxis) {
+            if (M > kMaxLen) {
                 delete P;
                 return TOPE_ERROR_INVALID_SIZE;
             }
@@ -1177,9 +1248,14 @@ int build_real_plan(struct topeFFT *f, long long nx, long long ny, long long nz,
     if (!real) return TOPE_ERROR_INVALID_ARG;
     if (nx < 2 || (nx & 1) || ny < 1 || nz < 1) return TOPE_ERROR_INVALID_SIZE;
     if (prec != PREC_F64 && prec != PREC_F32) return TOPE_ERROR_INVALID_ARG;
-    // each axis (x/2 for the rows) a power of two up to 2^24, or 7-smooth up
-    // to 4096 (one mixed-radix pass)
-    auto covered = [](long long n) { return is_pow2(n) ? n <= kMaxAxis : mixed_supported(n); };
+    // each axis (x/2 for the rows) a power of two (up to 2^30: one pass up to
+    // 4096, the four-step up to 2^24, three passes above), or 7-smooth up to
+    // 4096 (one mixed-radix pass) or above when it splits into two factors of
+    // at most 4096 (two mixed-radix passes)
+    auto covered = [](long long n) {
+        long long n1 = 0;
+        return is_pow2(n) ? n <= kMaxLen : (mixed_supported(n) || mixed4_split(n, &n1));
+    };
     if (!covered(nx / 2) || !covered(ny) || !covered(nz)) return TOPE_ERROR_UNSUPPORTED;
     const long long m = nx / 2, rows = ny * nz;
     if ((m + 1) * rows > (1LL << 33)) return TOPE_ERROR_INVALID_SIZE;
@@ -1227,8 +1303,14 @@ int build_real_plan(struct topeFFT *f, long long nx, long long ny, long long nz,
     std::vector<size_t> group_end;
     const bool fused = real_rows_supported(m);
     const Geo rows_geo{1, 0, rows, m, 1};
-    int rc = fused ? (int)TOPE_SUCCESS
-                   : (is_pow2(m) ? pl.axis(m, rows_geo, &cur, 0) : pl.mixed_axis(m, rows_geo, cur, 0));
+    // one axis of the real plan: power of two, 7-smooth in one mixed-radix
+    // pass, or the mixed-radix four-step (the data ends where it started)
+    auto real_axis = [&](long long n, const Geo &g, int id) {
+        if (is_pow2(n)) return pl.axis(n, g, &cur, id);
+        if (mixed_supported(n)) return pl.mixed_axis(n, g, cur, id);
+        return pl.mixed_four_step(n, g, &cur, id);
+    };
+    int rc = fused ? (int)TOPE_SUCCESS : real_axis(m, rows_geo, 0);
     group_end.push_back(P->passes.size());
     if (!rc && cur != 0) rc = TOPE_ERROR_UNSUPPORTED;
     if (!rc) {
@@ -1263,7 +1345,7 @@ int build_real_plan(struct topeFFT *f, long long nx, long long ny, long long nz,
         auto axis2 = [&](long long n, Geo g, int id) {
             if (n == 1) return (int)TOPE_SUCCESS;
             const size_t before = P->passes.size();
-            int r = is_pow2(n) ? pl.axis(n, g, &cur, id) : pl.mixed_axis(n, g, cur, id);
+            int r = real_axis(n, g, id);
             if (r) return r;
             for (size_t i = before; i < P->passes.size(); ++i) {
                 Pass &ps = P->passes[i];
