@@ -329,6 +329,26 @@ def small_1d_leg(fw, steps, warmup):
             "passes_us": [{"kernel": nm, "us": round(float(m) * 1e3, 2)} for nm, m in zip(names, pass_ms)]}
 
 
+def long_1d_leg(fw, steps, warmup):
+    """1D FP64 of 2^24 points (the four-step: two 4096-point passes) and
+    2^25 points (three passes of 512, 256, 256): device time per transform,
+    the rate of one HBM round trip of the array per pass, pass times."""
+    import topefft
+    import workloads
+    out = {}
+    for n in (1 << 24, 1 << 25):
+        x = workloads.complex_uniform(workloads.SEED_1D + n % 1000, n)
+        p = topefft.Plan(fw, (n,), x)
+        t, pass_ms = measure(fw, p, steps, warmup)
+        names = p.pass_names()
+        p.destroy()
+        del x
+        out[str(n)] = {"us_per_transform": round(t * 1e6, 1), "passes": len(names),
+                       "gbs_per_pass": round(gbs(2 * 16 * n * len(names), t), 1),
+                       "passes_us": [{"kernel": nm, "us": round(float(m) * 1e3, 1)} for nm, m in zip(names, pass_ms)]}
+    return out
+
+
 def vendor_leg(steps):
     """rocFFT (ROCm's FFT library, run-only comparator) on the same
     device-resident transforms, timed by tope-fft_amd/lib/vendor_compare
@@ -410,7 +430,7 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="uneven batch: skip the hipGraph-replay methods (for kernel-trace profiling)")
     ap.add_argument("--legs", default="all",
-                    help="comma list of mp2d,1d,3d256,mp,many,3d512,r2c,vendor,yard (default all; 2d is always run)")
+                    help="comma list of mp2d,1d,long,3d256,mp,many,3d512,r2c,vendor,yard (default all; 2d is always run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -438,13 +458,15 @@ def main():
     # the side legs (reported beside the headline, not part of `value`) time
     # at most 20 steps each so the default run stays short
     side_steps = max(5, min(20, args.steps // 2))
-    legs = set(args.legs.split(",")) if args.legs != "all" else {"mp2d", "1d", "3d256", "mp", "many", "3d512", "r2c",
-                                                                  "vendor", "yard"}
+    legs = set(args.legs.split(",")) if args.legs != "all" else {"mp2d", "1d", "long", "3d256", "mp", "many", "3d512",
+                                                                  "r2c", "vendor", "yard"}
     extra = {}
     if "mp2d" in legs:
         extra["mixed_precision_2d_4096"] = mp_2d_leg(fw, x2, t2, args.steps, args.warmup)
     if "1d" in legs:
         extra["1d_65536"] = small_1d_leg(fw, args.steps, args.warmup)
+    if "long" in legs:
+        extra["1d_long"] = long_1d_leg(fw, min(side_steps, 10), min(args.warmup, 3))
     # ---- 3D 256^3 FP64 (same recipe, beside it) ---------------------------
     if "3d256" in legs:
         dims3 = (256, 256, 256)

@@ -1,4 +1,4 @@
-"""Axes longer than 4096^2 points against the oracle.
+"""Axes of 4096^2 points and longer against the oracle.
 
 Power-of-two lengths above 2^24 run three passes (n = A B C, each factor
 at most 4096: a post-twiddled pass over the outer index, the four-step of
@@ -48,6 +48,15 @@ def check_1d(oracle, n, x, salt, tol=TOL, nbins=10):
     return names
 
 
+def test_1d_pow2_four_step_4096(oracle):
+    # 2^24 = 4096 x 4096: the first step runs the three-step column tile
+    # with the four-step post-twiddle (strided input and output), the
+    # second the one-shot column tile (contiguous input, transposed output)
+    n = 1 << 24
+    names = check_1d(oracle, n, seeded(n, 6), 6)
+    assert names == ["axis_double_n4096_e16_tb4_col_ct3", "axis_double_n4096_e16_tb4_col_split"], names
+
+
 def test_1d_pow2_three_pass(oracle):
     n = 1 << 25  # 512 x 256 x 256
     names = check_1d(oracle, n, seeded(n, 1), 1)
@@ -90,6 +99,26 @@ def test_batch_three_pass_complex64(oracle):
     bins = sample_bins(n, 40, 10)
     err = rel_l2(X[bins], oracle.dft_bins(x64, (n, 1, 1), bins, topefft.FORWARD))
     print(f"complex64 2^25 rel-L2 at sampled bins {err:.3e}")
+    assert err < 1e-6
+    assert rel_l2(back, x64) < 1e-6
+
+
+def test_batch_four_step_4096_complex64(oracle):
+    # complex64 2^24 = 4096 x 4096 (FP32 column tiles with the post-twiddle)
+    n = 1 << 24
+    x = seeded(n, 7).astype(np.complex64)
+    fw = topefft.Framework()
+    p = topefft.BatchPlan(fw, n, 1, x, topefft.PREC_F32, topefft.MODE_VALU)
+    names = p.pass_names()
+    X = p.exec(topefft.FORWARD).astype(np.complex128)
+    back = p.exec(topefft.INVERSE).astype(np.complex128)
+    p.destroy()
+    fw.destroy()
+    assert len(names) == 2 and all("float_n4096" in nm for nm in names), names
+    x64 = x.astype(np.complex128)
+    bins = sample_bins(n, 70, 10)
+    err = rel_l2(X[bins], oracle.dft_bins(x64, (n, 1, 1), bins, topefft.FORWARD))
+    print(f"complex64 2^24 rel-L2 at sampled bins {err:.3e}")
     assert err < 1e-6
     assert rel_l2(back, x64) < 1e-6
 
