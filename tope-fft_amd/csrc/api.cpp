@@ -534,10 +534,11 @@ class Planner {
         if (n > kMaxAxis) return three_step(n, g, cur, axis_id);
         // four-step: i = n1 + N1*n2.  Step 1 transforms n2 (length N2, stride
         // N1*is) per (line, n1), multiplying output k2 by W_n^{n1*k2}, into
-        // the other buffer; step 2 transforms n1 (length N1, stride is) per
-        // (line, k2) and writes X[k2 + N2*k1] back into this one, so the
-        // result ends where the data started (no swap, no copy for callers'
-        // arrays).
+        // the other buffer (in place when `flip`); step 2 transforms n1
+        // (length N1, stride is) per (line, k2) and writes X[k2 + N2*k1]
+        // into the buffer step 1 did not write, so without `flip` the result
+        // ends where the data started (no swap, no copy for callers' arrays)
+        // and with it in the other buffer.
         const int L = ilog2ll(n);
         const long long N1 = 1LL << ((L + 1) / 2), N2 = n / N1;
         const int lo_bits = (L + 1) / 2;
@@ -551,25 +552,30 @@ class Planner {
         Fold f1 = fold(g, N1, g.is, g.is);
         Geo s1i = f1.in, s1o = f1.out;
         s1i.is = s1o.is = N1 * g.is;
-        int rc = add_pass((int)N2, s1i, s1o, *cur, *cur ^ 1, f1.reps, f1.rep_in, f1.rep_out, lo, hi,
+        const int b1 = flip ? *cur : *cur ^ 1;
+        int rc = add_pass((int)N2, s1i, s1o, *cur, b1, f1.reps, f1.rep_in, f1.rep_out, lo, hi,
                           f1.new_in_o ? 1 : 0, f1.new_in_c ? 1 : 0, n, lo_bits, axis_id);
         if (rc) return rc;
-        // step 2 (back, transposed): lines are (old o, old c, k2)
+        // step 2 (transposed): lines are (old o, old c, k2)
         Fold f2 = fold(g, N2, N1 * g.is, g.is);
         Geo s2i = f2.in, s2o = f2.out;
         s2i.is = g.is;
         s2o.is = N2 * g.is;
-        return add_pass((int)N1, s2i, s2o, *cur ^ 1, *cur, f2.reps, f2.rep_in, f2.rep_out);
+        rc = add_pass((int)N1, s2i, s2o, b1, b1 ^ 1, f2.reps, f2.rep_in, f2.rep_out);
+        *cur = b1 ^ 1;
+        return rc;
     }
 
     // Power-of-two axis longer than 4096^2: n = A B C (each at most 4096),
     // i = a + A b + A B c, k = k_c + C k_b + C B k_a.  Three passes:
     //   1. DFT over c (stride A B is) for each (line, a + A b), output k_c
-    //      times W_n^{(a + A b) k_c}, into the other buffer, same layout;
+    //      times W_n^{(a + A b) k_c}, into the other buffer, same layout (in
+    //      place when `flip`);
     //   2. in place there: DFT over b (stride A is) for each (line, a, k_c),
     //      output k_b times W_{AB}^{a k_b};
-    //   3. DFT over a (stride is) for each (line, k_b, k_c), written back as
-    //      X[k_c + C k_b + C B k_a], so the data ends where it started.
+    //   3. DFT over a (stride is) for each (line, k_b, k_c), written to the
+    //      other buffer as X[k_c + C k_b + C B k_a]: without `flip` the data
+    //      ends where it started.
     // Steps 2 and 3 are the four-step of the inner A B-point transforms.
     // The axis' own lines may use one index (O or C) at most.
     int three_step(long long n, const Geo &g, int *cur, int axis_id)
@@ -598,18 +604,21 @@ class Planner {
         if (err != hipSuccess) return map_err(err);
         Geo i1 = f1.in, o1 = f1.out;
         i1.is = o1.is = AB * is;
-        int rc = add_pass((int)C, i1, o1, *cur, *cur ^ 1, f1.reps, f1.rep_in, f1.rep_out, lo1, hi1,
+        const int b1 = flip ? *cur : *cur ^ 1;
+        int rc = add_pass((int)C, i1, o1, *cur, b1, f1.reps, f1.rep_in, f1.rep_out, lo1, hi1,
                           f1.new_in_o ? 1 : 0, f1.new_in_c ? 1 : 0, n, h1, axis_id);
         if (rc) return rc;
         Geo i2 = f2.in, o2 = f2.out;
         i2.is = o2.is = A * is;
-        rc = add_pass((int)B, i2, o2, *cur ^ 1, *cur ^ 1, f2.reps, f2.rep_in, f2.rep_out, lo2, hi2,
+        rc = add_pass((int)B, i2, o2, b1, b1, f2.reps, f2.rep_in, f2.rep_out, lo2, hi2,
                       f2.new_in_o ? 1 : 0, f2.new_in_c ? 1 : 0, AB, h2);
         if (rc) return rc;
         Geo i3 = f3.in, o3 = f3.out;
         i3.is = is;
         o3.is = C * B * is;
-        return add_pass((int)A, i3, o3, *cur ^ 1, *cur, f3.reps, f3.rep_in, f3.rep_out);
+        rc = add_pass((int)A, i3, o3, b1, b1 ^ 1, f3.reps, f3.rep_in, f3.rep_out);
+        *cur = b1 ^ 1;
+        return rc;
     }
 
     // Non-power-of-two axis of length n (lines g) by Bluestein's identity
@@ -866,6 +875,15 @@ class Planner {
         *cur ^= 1;
         return TOPE_SUCCESS;
     }
+
+    // true: an axis longer than 4096 may leave its data in the other buffer
+    // (complex plans swap their buffers at the end); its first pass then
+    // runs in place, which measured 102 us against 195 us for the 2^24
+    // four-step's first step (a strided pass writing 64-byte segments into
+    // a buffer it did not just read loses the Infinity Cache's merging of
+    // the two halves of each 128-byte line).  The real-input plans keep
+    // each axis's data where it started.
+    bool flip = false;
 
    private:
     PlanImpl *P;
@@ -1126,6 +1144,7 @@ int build_plan(struct topeFFT *f, int shape, long long nx, long long ny, long lo
     if (err != hipSuccess) return fail(map_err(err));
 
     Planner pl(P, s);
+    pl.flip = true;  // the plan swaps its buffers when the data ends in buffer 1
     int cur = 0, rc = TOPE_SUCCESS;
     if (shape == SHAPE_BATCH && mode != MP_VALU) {
         // one MFMA launch over all rows, in place

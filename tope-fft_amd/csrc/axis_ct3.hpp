@@ -38,11 +38,15 @@ namespace tope {
 // against 100.4 us for the 2D 4096^2 pass.
 constexpr int CT3_TW = 128 + 128;
 
-template <typename R>
+// Four-step post-twiddle factors of a tile (POST): for each column t of
+// base b_t, W^{b_t g} (g < 16), W^{16 b_t w} (w < 16) and W^{256 b_t}.
+constexpr int CT3_PT = 4 * 16 + 4 * 16 + 4;
+
+template <typename R, bool POST = false>
 struct Ct3Lds {
     static constexpr int WR = 16 * 68;                   // values of one component per wave region
     static constexpr int XB = 16 * WR * (int)sizeof(R);  // 16 wave regions
-    static constexpr int BYTES = XB + CT3_TW * (int)sizeof(cpx<R>);
+    static constexpr int BYTES = XB + (CT3_TW + (POST ? CT3_PT : 0)) * (int)sizeof(cpx<R>);
 };
 
 // Orders a wave's LDS writes before its reads of other lanes' values (and
@@ -60,7 +64,7 @@ __global__ void __launch_bounds__(1024) ct3_kernel(const cpx<R> *in, cpx<R> *out
 {
     constexpr int TB = 4;
     constexpr int CB = (int)sizeof(cpx<R>);
-    using L = Ct3Lds<R>;
+    using L = Ct3Lds<R, POST>;
     __shared__ __align__(16) unsigned char lds[L::BYTES];
     R *xl = reinterpret_cast<R *>(lds);
     cpx<R> *ltw = reinterpret_cast<cpx<R> *>(lds + L::XB);
@@ -77,6 +81,18 @@ __global__ void __launch_bounds__(1024) ct3_kernel(const cpx<R> *in, cpx<R> *out
     const int c0 = tile * TB;
     const int c = c0 + t;
     const bool valid = c < a.C;
+    // the post-twiddle factors of the tile's columns, read from the tables
+    // while the data loads are in flight (at the end of the tile, after the
+    // last barrier, table reads from L2 wait behind the streaming traffic)
+    cpx<R> *ptl = ltw + CT3_TW;
+    if constexpr (POST) {
+        if (tid < CT3_PT) {
+            const int ct = tid < 128 ? (tid >> 4) & 3 : tid - 128;
+            const long long bt = o * a.pt_o + (long long)(c0 + ct) * a.pt_c;
+            const long long x = tid < 64 ? bt * (tid & 15) : tid < 128 ? bt * 16 * (tid & 15) : bt * 256;
+            ptl[tid] = pt_lookup(x, a.pt_lo, a.pt_hi, a.pt_n, a.pt_shift);
+        }
+    }
 
     // step A inputs: rows 256 e + 16 g + w of column c
     cpx<R> v[16];
@@ -156,9 +172,8 @@ __global__ void __launch_bounds__(1024) ct3_kernel(const cpx<R> *in, cpx<R> *out
 
     if (!valid) return;
     // output rows k = g + 16 w + 256 e
-    if constexpr (POST)
-        post_twiddle<SIGN>(v, o * a.pt_o + (long long)c * a.pt_c, g + 16 * w, 256, a.pt_lo, a.pt_hi, a.pt_n,
-                           a.pt_shift);
+    if constexpr (POST)  // W^{b k}, k = g + 16 w + 256 e: W^{b g} W^{16 b w} (W^{256 b})^e
+        pt_chain<SIGN>(v, cmul(ptl[16 * t + g], ptl[64 + 16 * t + w]), ptl[128 + t]);
     if (a.scale != R(1)) {
 #pragma unroll
         for (int e = 0; e < 16; ++e) v[e] = scale(v[e], a.scale);

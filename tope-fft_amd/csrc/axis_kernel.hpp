@@ -185,28 +185,47 @@ __device__ __forceinline__ void buf_store(cpx<R> v, __amdgpu_buffer_rsrc_t r, in
         __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(tope_u32x2, v), r, voff, soff, 0);
 }
 
+// W_{pt_n}^x = pt_lo[x & mask] * pt_hi[x >> pt_shift] (x reduced mod pt_n).
+template <typename R>
+__device__ __forceinline__ cpx<R> pt_lookup(long long x, const cpx<R> *pt_lo, const cpx<R> *pt_hi, long long pt_n,
+                                            int pt_shift)
+{
+    x &= pt_n - 1;
+    return cmul(pt_lo[x & ((1LL << pt_shift) - 1)], pt_hi[x >> pt_shift]);
+}
+
+// v[e] *= w s^e: a chain of products of the step s (relative error ~E ulp
+// in FP64).
+template <int SIGN, typename R, int E>
+__device__ __forceinline__ void pt_chain(cpx<R> (&v)[E], cpx<R> w, cpx<R> s)
+{
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        if (e) w = cmul(w, s);
+        v[e] = twmul<SIGN>(v[e], w);
+    }
+}
+
 // Four-step post-twiddle of one thread's outputs k = p + P e (e < E) of a
 // line with twiddle base b: v[e] *= W_{pt_n}^{b k}.  W^{b (p + P e)} is
 // W^{b p} (W^{b P})^e: a fresh table value every RS outputs, products of
 // the step W^{b P} in between (FP64: one chain of E - 1 products, relative
-// error ~E ulp; FP32: fresh every 4 outputs).  W_{pt_n}^a is
-// pt_lo[a & mask] * pt_hi[a >> pt_shift]; b k < 2^63.
+// error ~E ulp; FP32: fresh every 4 outputs); b k < 2^63.
 template <int SIGN, typename R, int E>
 __device__ __forceinline__ void post_twiddle(cpx<R> (&v)[E], long long b, long long p, long long P,
                                              const cpx<R> *pt_lo, const cpx<R> *pt_hi, long long pt_n, int pt_shift)
 {
     constexpr int RS = sizeof(R) == 8 ? E : 4;
-    const long long nm = pt_n - 1, lm = (1LL << pt_shift) - 1;
-    auto look = [&](long long ex) {
-        ex &= nm;
-        return cmul(pt_lo[ex & lm], pt_hi[ex >> pt_shift]);
-    };
-    const cpx<R> s = look(b * P);
-    cpx<R> w = look(b * p);
+    if constexpr (RS == E) {
+        pt_chain<SIGN>(v, pt_lookup(b * p, pt_lo, pt_hi, pt_n, pt_shift), pt_lookup(b * P, pt_lo, pt_hi, pt_n, pt_shift));
+    } else {
+        const cpx<R> s = pt_lookup(b * P, pt_lo, pt_hi, pt_n, pt_shift);
+        cpx<R> w = pt_lookup(b * p, pt_lo, pt_hi, pt_n, pt_shift);
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-        if (e) w = (e % RS == 0) ? look(b * (p + P * e)) : cmul(w, s);
-        v[e] = twmul<SIGN>(v[e], w);
+        for (int e = 0; e < E; ++e) {
+            if (e) w = (e % RS == 0) ? pt_lookup(b * (p + P * e), pt_lo, pt_hi, pt_n, pt_shift) : cmul(w, s);
+            v[e] = twmul<SIGN>(v[e], w);
+        }
     }
 }
 

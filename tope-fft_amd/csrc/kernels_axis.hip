@@ -111,7 +111,7 @@ constexpr int prec_of() { return sizeof(R) == 8 ? PREC_F64 : PREC_F32; }
 
 // Three-step column tile (4096 points, 4 columns, 16 elements per thread).
 #define VARC(R)                                                                                               \
-    AxisVariant{prec_of<R>(), 4096, 16, 4, true, true, false, false, true, 1024, CT3_TW, Ct3Lds<R>::BYTES, \
+    AxisVariant{prec_of<R>(), 4096, 16, 4, true, true, false, false, true, 1024, CT3_TW, Ct3Lds<R, true>::BYTES, \
                 TOPE_LAUNCH_CT3(R), "axis_" TOPE_STR(R) "_n4096_e16_tb4_col_ct3"}
 
 // Even/odd-split column tiles for the long strided axes.
