@@ -12,7 +12,8 @@
 //            elements per thread (the 2D 4096^2 X pass's pattern)
 //   cols     in place, one block of 1024 threads per tile of 4 adjacent
 //            columns of 4096 rows, lanes walk the 4 columns first, blocks
-//            laid out per XCD (the Y pass's pattern)
+//            laid out per XCD as the axis passes lay them out (the Y pass's
+//            pattern)
 //   all      every mode above
 // `work` adds that many rounds of two fused multiply-adds per element
 // between the loads and the stores (a stand-in for a pass's arithmetic;
@@ -75,7 +76,8 @@ __global__ void __launch_bounds__(1024) cols_copy(double2 *a, int work, double f
 {
     const long long b = blockIdx.x, G = gridDim.x;
     const long long q = G >> 3, r = G & 7, x = b & 7, k = b >> 3;
-    const long long tile = x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+    const long long tile = (G & 255) == 0 ? ((k >> 5) << 8) + (x << 5) + (k & 31)
+                                          : (x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k);
     const int t = threadIdx.x & 3, p = threadIdx.x >> 2;
     double2 *col = a + tile * 4 + t;
     double2 v[16];
