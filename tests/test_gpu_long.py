@@ -130,6 +130,50 @@ def test_1d_bluestein_three_pass(oracle):
     assert "chirp_pre" in names and "chirp_post" in names, names
 
 
+def test_2d_column_tile_64bit_addressing(oracle):
+    # 2D 40000 x 4096 (2.4 GiB FP64; the rows run the mixed-radix
+    # four-step, 200 x 200): the 4096-point column pass spans more than
+    # 2^31 bytes per tile (rows 625 KiB apart), so it runs the three-step
+    # tile's 64-bit addressing instantiation
+    dims = (40000, 4096)
+    n = dims[0] * dims[1]
+    x = seeded(n, 9)
+    fw = topefft.Framework()
+    p = topefft.Plan(fw, dims, x)
+    names = p.pass_names()
+    X = p.exec(topefft.FORWARD)
+    back = p.exec(topefft.INVERSE)
+    p.destroy()
+    fw.destroy()
+    assert names[-1] == "axis_double_n4096_e16_tb4_col_ct3", names
+    bins = sample_bins(n, 90, 4)
+    assert rel_l2(X[bins], oracle.dft_bins(x, (dims[0], dims[1], 1), bins, topefft.FORWARD)) < TOL
+    assert rel_l2(back, x) < TOL
+
+
+def test_exec_ptr_three_pass_matches_plan_exec():
+    # topePlanExecPtr on a caller array (a torch tensor): the three passes
+    # run in place on it and end in the plan's second array, then one copy
+    # brings the result back; bit-identical to the plan's own exec
+    torch = pytest.importorskip("torch")
+    n = 1 << 25
+    x = seeded(n, 8)
+    fw = topefft.Framework()
+    p = topefft.Plan(fw, (n,), x)
+    want = p.exec(topefft.FORWARD)
+    q = topefft.Plan(fw, (n,), np.zeros(n, complex))
+    t = torch.from_numpy(x.copy()).to("cuda")
+    torch.cuda.synchronize()
+    q.exec_ptr(t.data_ptr(), topefft.FORWARD)
+    fw.synchronize()
+    got = t.cpu().numpy()
+    assert not np.any(q.download())  # the plan's own array is untouched
+    q.destroy()
+    p.destroy()
+    fw.destroy()
+    assert np.array_equal(got, want)
+
+
 def test_refuses_beyond_the_three_pass_limit():
     # a Bluestein row of 2^31 points (n = 2^29 + 1) is refused at planning,
     # before anything is allocated or the input is read (the buffer passed
