@@ -65,10 +65,16 @@ __device__ __forceinline__ long long divmod64(long long a, const Div64 &d, long 
     return q;
 }
 
+// The block's copy of the roots in LDS, in two levels: W_n^i = hi[i >> 6]
+// lo[i & 63] with lo[j] = W_n^j (j < 64) and hi[j] = W_n^{64 j}; 128 entries
+// whatever n <= 4096 (the per-butterfly reads of the global W_n table cost
+// 20 % of a 3000-point pass).
+constexpr int ROOT_LO = 64;
+constexpr int ROOT_SLOTS = ROOT_LO + MIXED_MAX_N / ROOT_LO;
 template <int SIGN, typename R>
 __device__ __forceinline__ cpx<R> root(const cpx<R> *__restrict__ w, int i)
 {
-    const cpx<R> v = w[i];
+    const cpx<R> v = cmul(w[ROOT_LO + (i >> 6)], w[i & (ROOT_LO - 1)]);
     return SIGN < 0 ? v : cpx<R>{v.x, -v.y};
 }
 
@@ -253,14 +259,27 @@ __device__ __forceinline__ void mixed_stage_r(int r, const cpx<R> *src, cpx<R> *
 // threads: 512 for CAP = 2048 (two blocks, 16 waves per CU: 4 elements per
 // thread keep the latency of the stage loop covered; 256 threads measured
 // 10-12 % slower at 100^3 and 125^3), 1024 for the single-line 4096-element
-// blocks that own a CU.
+// blocks that own a CU.  Four waves per SIMD: the FP64 kernel needs 162
+// VGPRs on its own (one 512-thread block per CU); capped at 128 (with
+// spills) two blocks share a CU, 9-21 % faster (1000^2 44.0 -> 37.1 us,
+// 3000^2 360 -> 328 us with the LDS roots, 1D 10^6 46.5 -> 36.7 us).
 template <typename R, int SIGN, int CAP, int NT>
-__global__ void __launch_bounds__(NT) mixed_kernel(const cpx<R> *in, cpx<R> *out, const MixedArgs a)
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4)))
+mixed_kernel(const cpx<R> *in, cpx<R> *out, const MixedArgs a)
 {
     constexpr int SLOTS = CAP + CAP / 16;
-    __shared__ cpx<R> lds[2 * SLOTS];
-    const cpx<R> *roots = static_cast<const cpx<R> *>(a.roots);
+    __shared__ cpx<R> lds[2 * SLOTS + ROOT_SLOTS];
     const int n = a.n, tb = a.tb;
+    // the two-level roots (read from the second stage on, after a barrier)
+    cpx<R> *roots = lds + 2 * SLOTS;
+    {
+        const cpx<R> *g = static_cast<const cpx<R> *>(a.roots);
+        const int nhi = (n + ROOT_LO - 1) / ROOT_LO;
+        for (int i = threadIdx.x; i < ROOT_LO + nhi; i += NT) {
+            const int e = i < ROOT_LO ? i : (i - ROOT_LO) * ROOT_LO;
+            roots[i] = e < n ? g[e] : cpx<R>{1, 0};
+        }
+    }
     const int span_elems = tb * n;
     const bool cfast = a.is != 1;  // strided axis: neighbouring lanes take neighbouring lines
     const bool cfast_out = a.ois != 1;  // the same for the last stage's stores
