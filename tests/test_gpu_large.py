@@ -85,3 +85,48 @@ def test_512_complex64_against_fp64(field, mode, kernels):
     print(f"512^3 complex64 {names[0]}: rel-L2 vs FP64 {err:.3e}, round trip {rt:.3e}")
     assert err < 1e-6
     assert rt < 1e-6
+
+
+# 2D 4096^2 (the benchmark shape): the complex64 plans' 4096-point column
+# pass runs the paired-column three-step tile (8 columns, two per thread)
+# under every mode; their spectra against the FP64 plan's (pinned to the
+# direct DFT by the full-size parity test of this shape), both directions.
+DIMS2 = (4096, 4096)
+
+
+@pytest.fixture(scope="module")
+def field2d():
+    x, _ = workloads.planted_input(workloads.SEED_2D, DIMS2 + (1,))
+    fw = topefft.Framework()
+    p = topefft.Plan(fw, DIMS2, x)
+    names = p.pass_names()
+    X = p.exec(topefft.FORWARD)
+    Xi = p.exec(topefft.INVERSE)  # of the spectrum: x again
+    p.destroy()
+    fw.destroy()
+    assert names == ["axis_double_n4096_e16_tb1_row_split", "axis_double_n4096_e16_tb4_col_ct3"], names
+    return x, X, Xi
+
+
+MODES2 = [(None, "axis_float_n4096_e16_tb1_row_split"), (topefft.MODE_MFMA_FP16X3, "mfma_rows_fp16x3"),
+          (topefft.MODE_MFMA_BF16X6, "mfma_rows_bf16x6")]
+
+
+@pytest.mark.parametrize("mode,rows", MODES2, ids=["valu", "mfma_fp16x3", "mfma_bf16x6"])
+def test_4096_complex64_pair_tile(field2d, mode, rows):
+    x, X, Xi = field2d
+    x32 = x.astype(np.complex64)
+    fw = topefft.Framework()
+    q = topefft.Plan(fw, DIMS2, x32, precision=topefft.PREC_F32, mode=mode)
+    names = q.pass_names()
+    X32 = q.exec(topefft.FORWARD).astype(np.complex128)
+    back = q.exec(topefft.INVERSE).astype(np.complex128)
+    q.destroy()
+    fw.destroy()
+    assert names == [rows, "axis_float_n4096_e16_tb8_col_ct3"], names
+    err = rel_l2(X32, X)
+    rt = rel_l2(back, x32.astype(np.complex128))
+    print(f"4096^2 complex64 {names}: rel-L2 vs FP64 {err:.3e}, round trip {rt:.3e}")
+    assert err < 1e-6
+    assert rt < 1e-6
+    assert rel_l2(back, Xi) < 1e-6

@@ -62,6 +62,37 @@ def test_exec_ptr_complex64(oracle, torch):
     assert rel_l2(X, oracle.dft(x.astype(np.complex128), dims, topefft.FORWARD)) < 1e-6
 
 
+def test_exec_ptr_complex64_column_pairs(oracle, torch, monkeypatch):
+    # the complex64 three-step column tile moves two adjacent columns as one
+    # 16-byte element: on a caller array it matches the plan's own exec bit
+    # for bit, and an array that is only 8-byte aligned is refused before
+    # anything is enqueued
+    monkeypatch.setenv("TOPEFFT_VARIANT_4096_col", "axis_float_n4096_e16_tb8_col_ct3")
+    dims = (16, 4096)
+    n = 16 * 4096
+    rng = np.random.default_rng(16)
+    x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    fw = topefft.Framework()
+    p = topefft.Plan(fw, dims, x, precision=topefft.PREC_F32)
+    assert p.pass_names()[-1] == "axis_float_n4096_e16_tb8_col_ct3", p.pass_names()
+    want = p.exec(topefft.FORWARD)
+    t = torch.from_numpy(np.concatenate([x, np.zeros(1, np.complex64)])).to("cuda")
+    torch.cuda.synchronize()
+    p.exec_ptr(t.data_ptr(), topefft.FORWARD)
+    fw.synchronize()
+    got = t.cpu().numpy()
+    with pytest.raises(topefft.TopeError) as e:
+        p.exec_ptr(t.data_ptr() + 8, topefft.FORWARD)
+    assert e.value.code == topefft.TOPE_ERROR_INVALID_ARG
+    fw.synchronize()
+    assert np.array_equal(t.cpu().numpy(), got)  # nothing ran
+    p.destroy()
+    fw.destroy()
+    assert np.array_equal(got[:n], want)
+    assert got[n] == 0
+    assert rel_l2(want.astype(np.complex128), oracle.dft(x.astype(np.complex128), dims, topefft.FORWARD)) < 1e-6
+
+
 def test_exec_ptr_errors():
     fw = topefft.Framework()
     p = topefft.Plan(fw, (16, 16), np.zeros(256, complex))

@@ -82,7 +82,8 @@ def test_every_preferred_variant_is_compiled():
                  "axis_double_n2048_e16_tb4_col_split", "axis_double_n256_e16_tb32_col_split",
                  "axis_double_n512_e16_tb16_col_split", "axis_double_n256_e16_tb4_row",
                  "axis_float_n512_e16_tb32_col_split", "axis_float_n256_e16_tb32_col_split",
-                 "axis_float_n4096_e16_tb1_row_split", "axis_float_n256_e16_tb4_row"):
+                 "axis_float_n4096_e16_tb1_row_split", "axis_float_n256_e16_tb4_row",
+                 "axis_float_n4096_e16_tb8_col_ct3"):
         assert want in names, want
 
 

@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -96,6 +97,7 @@ struct PlanImpl {
     size_t buf_bytes = 0;  // device arrays when larger than bytes (Bluestein's padded length)
     void *buf[2] = {nullptr, nullptr};
     void *ct3_tw = nullptr;  // twiddle table of the three-step column tiles
+    bool pair_tiles = false;  // a pass moves complex64 column pairs (16-byte aligned arrays)
     int final_buf = 0;
     std::vector<Pass> passes;
     std::vector<void *> allocs;
@@ -206,6 +208,8 @@ const Preferred kPreferred[] = {
     {"axis_double_n256_e16_tb32_col_split", kMinBlocks},
     {"axis_double_n512_e16_tb16_col_split", kMinBlocks},
     {"axis_double_n256_e16_tb4_row", 8192},
+    // complex64 4096-point columns in pairs (2D 4096^2 Y pass)
+    {"axis_float_n4096_e16_tb8_col_ct3", kMinBlocks, true},
     {"axis_float_n512_e16_tb32_col_split", kMinBlocks},
     {"axis_float_n256_e16_tb32_col_split", kMinBlocks},
     {"axis_float_n4096_e16_tb1_row_split", kMinBlocks},
@@ -220,20 +224,27 @@ const Preferred kPreferred[] = {
 // plain_only: one-shot variants only (no even/odd split or three-step
 // tile, which need tables of their own), for callers that provide no tables.
 // strided_io: the line elements are strided in the input and the output.
+// pair_ok: the complex64 three-step tile's conditions hold (adjacent
+// columns, an even column count, even strides, no post-twiddle).
 const AxisVariant *select_variant(int prec, int n, bool cfast, long long lines, bool dif2 = false,
-                                  long long outer = 1, bool plain_only = false, bool strided_io = false)
+                                  long long outer = 1, bool plain_only = false, bool strided_io = false,
+                                  bool pair_ok = false)
 {
+    auto pair_tile = [](const AxisVariant *v) { return v->ct3 && v->prec == PREC_F32; };
     char key[64];
     std::snprintf(key, sizeof key, "TOPEFFT_VARIANT_%d_%s%s", n, cfast ? "col" : "row", dif2 ? "_dif2" : "");
     if (const char *ov = std::getenv(key)) {
         const AxisVariant *v = find_variant_named(ov);
-        if (v && v->prec == prec && v->n == n && v->dif2 == dif2 && !(plain_only && (v->dit2 || v->ct3))) return v;
+        if (v && v->prec == prec && v->n == n && v->dif2 == dif2 && !(plain_only && (v->dit2 || v->ct3)) &&
+            !(pair_tile(v) && !pair_ok))
+            return v;
     }
     // blocks a tile width gives for this shape (small shapes: keep the CUs busy)
     auto blocks = [&](const AxisVariant *v) { return outer * ((lines + v->tb - 1) / v->tb); };
     for (const Preferred &pr : kPreferred) {
         const AxisVariant *v = find_variant_named(pr.name);
         if (v && plain_only && (v->dit2 || v->ct3)) continue;
+        if (v && pair_tile(v) && !pair_ok) continue;
         if (pr.strided_only && !strided_io) continue;
         if (v && v->prec == prec && v->n == n && v->cfast == cfast && v->dif2 == dif2 && v->tb <= lines &&
             blocks(v) >= pr.min_blocks)
@@ -403,8 +414,14 @@ class Planner {
     {
         const bool dif2 = pair != 0;
         const bool cfast = !(gi.is == 1 && go.is == 1);
-        const AxisVariant *v =
-            select_variant(P->prec, n, cfast, dif2 ? 2 * gi.C : gi.C, dif2, gi.O * reps, false, gi.is != 1 && go.is != 1);
+        // the complex64 three-step tile moves two adjacent columns as one
+        // 16-byte element: every pair must start 16-byte aligned
+        auto even = [](long long x) { return (x & 1) == 0; };
+        const bool pair_ok = !pt_lo && !dif2 && gi.cs == 1 && go.cs == 1 && even(gi.C) && even(gi.is) &&
+                             even(go.is) && (gi.O == 1 || (even(gi.os) && even(go.os))) &&
+                             (reps == 1 || (even(rep_in) && even(rep_out)));
+        const AxisVariant *v = select_variant(P->prec, n, cfast, dif2 ? 2 * gi.C : gi.C, dif2, gi.O * reps, false,
+                                              gi.is != 1 && go.is != 1, pair_ok);
         if (!v) return TOPE_ERROR_NO_KERNEL;
         // even/odd-split tiles run the stages of an n/2-point transform and
         // combine the halves with W_n^k, k < n/2 (passed as dif_tw)
@@ -452,6 +469,7 @@ class Planner {
         Pass ps;
         ps.var = v;
         ps.name = v->name;
+        if (v->ct3 && v->prec == PREC_F32) P->pair_tiles = true;
         AxisArgs<double> &a = ps.a;
         a.in_os = gi.os;
         a.in_cs = gi.cs;
@@ -2140,6 +2158,11 @@ void topePlanExecPtr(struct topeFFT *f, void *plan_impl, void *data, int dir)
     if (!data || P->real || P->buf_bytes > P->bytes) {
         // real plans and padded (Bluestein) plans work on arrays of their own
         f->error = !data ? TOPE_ERROR_INVALID_ARG : TOPE_ERROR_UNSUPPORTED;
+        return;
+    }
+    if (P->pair_tiles && (reinterpret_cast<uintptr_t>(data) & 15)) {
+        // complex64 column pairs are moved as 16-byte elements
+        f->error = TOPE_ERROR_INVALID_ARG;
         return;
     }
     // run the plan's passes with the caller's array in place of its own;

@@ -192,4 +192,176 @@ __global__ void __launch_bounds__(1024) ct3_kernel(const cpx<R> *in, cpx<R> *out
     }
 }
 
+// Complex64 three-step tile: 8 columns (64-byte row segments, as the FP64
+// tile's 4), each thread holding the two adjacent columns 2t, 2t + 1 of its
+// lane position as one 16-byte element per row.  The steps, twiddles and
+// exchanges are those of ct3_kernel, run once per column of the pair, with
+// the twiddles shared; each exchange moves whole complex values (8-byte LDS
+// accesses, the FP64 tile's bank pattern) in one round per column.
+// Requires unit column stride, C a multiple of 8 (whole pairs and tiles),
+// even row and plane strides (16-byte aligned pairs) and no post-twiddle.
+struct Ct3PairLds {
+    static constexpr int WR = 16 * 68;  // complex slots per wave region
+    static constexpr int XB = 16 * WR * (int)sizeof(cpx<float>);
+    static constexpr int BYTES = XB + CT3_TW * (int)sizeof(cpx<float>);
+};
+
+// One row of a column pair as four dwords (re, im of column 2t, then 2t + 1).
+// The elements go through scalar values: __builtin_bit_cast applied to an
+// ext_vector element (u.y) reads the vector's first element with this
+// compiler (the IR loads offset 0 for every element).
+__device__ __forceinline__ void pair_split(tope_u32x4 u, cpx<float> &a, cpx<float> &b)
+{
+    const unsigned u0 = u[0], u1 = u[1], u2 = u[2], u3 = u[3];
+    a.x = __uint_as_float(u0);
+    a.y = __uint_as_float(u1);
+    b.x = __uint_as_float(u2);
+    b.y = __uint_as_float(u3);
+}
+
+__device__ __forceinline__ tope_u32x4 pair_join(cpx<float> a, cpx<float> b)
+{
+    return tope_u32x4{__float_as_uint(a.x), __float_as_uint(a.y), __float_as_uint(b.x), __float_as_uint(b.y)};
+}
+
+// In-wave exchange of one column (register k0 -> lane group k0, lane group
+// n1 -> register n1) through the wave's region `mine`.  Force-inlined: the
+// register arrays must stay in registers (an outlined call passes them
+// through the stack).
+__device__ __forceinline__ void ct3_pair_in_wave(cpx<float> (&v)[16], cpx<float> *mine, int g, int t)
+{
+    const int wb = 4 * g + t, rb = g * 68 + t;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) mine[k * 68 + wb] = v[k];
+    wave_lds_sync();
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = mine[rb + 4 * r];
+    wave_lds_sync();
+}
+
+// Cross-wave exchange of one column (register k1 -> wave k1, wave n0 ->
+// register n0): wave w writes row k of its own region, wave k reads row k
+// of every region.
+__device__ __forceinline__ void ct3_pair_cross(cpx<float> (&v)[16], cpx<float> *mine, const cpx<float> *xl, int w,
+                                               int lane)
+{
+#pragma unroll
+    for (int k = 0; k < 16; ++k) mine[k * 64 + lane] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = xl[r * Ct3PairLds::WR + w * 64 + lane];
+}
+
+template <int SIGN, bool BIG>
+__global__ void __launch_bounds__(1024) ct3_pair_kernel(const cpx<float> *in, cpx<float> *out,
+                                                        const AxisArgs<float> a)
+{
+    using R = float;
+    using L = Ct3PairLds;
+    constexpr int TB = 8;
+    constexpr int CB = (int)sizeof(cpx<R>);
+    __shared__ __align__(16) unsigned char lds[L::BYTES];
+    cpx<R> *xl = reinterpret_cast<cpx<R> *>(lds);
+    cpx<R> *ltw = reinterpret_cast<cpx<R> *>(lds + L::XB);
+
+    const int tid = threadIdx.x;
+    const int t = tid & 3, g = (tid >> 2) & 15, w = tid >> 6;
+    if (tid < CT3_TW) ltw[tid] = a.tw[tid];
+
+    const long long blk = xcd_block(a.xcd);
+    const long long o = blk / a.tiles;
+    int tile = (int)(blk - o * a.tiles) + a.rot;
+    if (tile >= a.tiles) tile -= a.tiles;
+    const int c0 = tile * TB;
+    const int c = c0 + 2 * t;
+    const bool valid = c < a.C;
+
+    // step A inputs: rows 256 e + 16 g + w of columns c, c + 1
+    cpx<R> v0[16], v1[16];
+    if (valid) {
+        if constexpr (!BIG) {
+            const long long ibase = o * a.in_os + (long long)c0;
+            const __amdgpu_buffer_rsrc_t rin = make_rsrc(in + ibase, (a.in_elems - ibase) * CB);
+            const int voff = (int)((2 * t + (long long)(16 * g + w) * a.in_is) * CB);
+            const int sstep = (int)(256LL * a.in_is * CB);
+#pragma unroll
+            for (int e = 0; e < 16; ++e)
+                pair_split(__builtin_amdgcn_raw_buffer_load_b128(rin, voff, e * sstep, 0), v0[e], v1[e]);
+        } else {
+            const tope_u32x4 *src = reinterpret_cast<const tope_u32x4 *>(in + o * a.in_os + (long long)c);
+            const long long st = a.in_is / 2;  // rows in pairs (even stride)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) pair_split(src[(long long)(256 * e + 16 * g + w) * st], v0[e], v1[e]);
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            v0[e] = cpx<R>{0, 0};
+            v1[e] = cpx<R>{0, 0};
+        }
+    }
+
+    dft16<SIGN>(v0);  // register k0
+    dft16<SIGN>(v1);
+
+    cpx<R> *mine = xl + w * L::WR;  // this wave's region
+    // in-wave exchange, one column at a time
+    ct3_pair_in_wave(v0, mine, g, t);
+    ct3_pair_in_wave(v1, mine, g, t);
+
+    // step B: register n1 times W_256^{n1 k0} (k0 = g), DFT over n1 -> register k1
+    {
+        cpx<R> tw[16];
+#pragma unroll
+        for (int m = 1; m < 16; m <<= 1) tw[m] = a.tw[m * g];
+        apply_twiddles<SIGN>(v0, tw);  // forms every tw[m]
+#pragma unroll
+        for (int m = 1; m < 16; ++m) v1[m] = twmul<SIGN>(v1[m], tw[m]);
+    }
+    dft16<SIGN>(v0);
+    dft16<SIGN>(v1);
+
+    // cross-wave exchange, one column at a time
+    ct3_pair_cross(v0, mine, xl, w, 4 * g + t);
+    __syncthreads();  // every wave has read the first column
+    ct3_pair_cross(v1, mine, xl, w, 4 * g + t);
+
+    // step C: register n0 times W_4096^{n0 b}, b = g + 16 w; DFT over n0 -> register k2
+    {
+        cpx<R> tw[16];
+#pragma unroll
+        for (int m = 1; m < 16; m <<= 1) tw[m] = cmul(ltw[128 + m * g], ltw[m * w]);
+        apply_twiddles<SIGN>(v0, tw);
+#pragma unroll
+        for (int m = 1; m < 16; ++m) v1[m] = twmul<SIGN>(v1[m], tw[m]);
+    }
+    dft16<SIGN>(v0);
+    dft16<SIGN>(v1);
+
+    if (!valid) return;
+    if (a.scale != R(1)) {
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            v0[e] = scale(v0[e], a.scale);
+            v1[e] = scale(v1[e], a.scale);
+        }
+    }
+    // output rows k = g + 16 w + 256 e
+    if constexpr (!BIG) {
+        const long long obase = o * a.out_os + (long long)c0;
+        const __amdgpu_buffer_rsrc_t rout = make_rsrc(out + obase, (a.out_elems - obase) * CB);
+        const int voff = (int)((2 * t + (long long)(g + 16 * w) * a.out_is) * CB);
+        const int sstep = (int)(256LL * a.out_is * CB);
+        // whole offset in the VGPR, soffset 0 (the store-data hazard note at buf_store)
+#pragma unroll
+        for (int e = 0; e < 16; ++e)
+            __builtin_amdgcn_raw_buffer_store_b128(pair_join(v0[e], v1[e]), rout, voff + e * sstep, 0, 0);
+    } else {
+        tope_u32x4 *dst = reinterpret_cast<tope_u32x4 *>(out + o * a.out_os + (long long)c);
+        const long long st = a.out_is / 2;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) dst[(long long)(g + 16 * w + 256 * e) * st] = pair_join(v0[e], v1[e]);
+    }
+}
+
 }  // namespace tope

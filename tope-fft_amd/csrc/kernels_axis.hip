@@ -77,11 +77,25 @@ static void launch_ct3_t(const void *in, void *out, const void *args, unsigned g
     }
 }
 
+// Complex64 three-step tiles (ct3_pair_kernel): 4096 points, 8 columns in
+// pairs, 1024 threads.
+template <bool BIG>
+static void launch_ct3p_t(const void *in, void *out, const void *args, unsigned grid, hipStream_t s, int sign)
+{
+    const AxisArgs<float> &a = *static_cast<const AxisArgs<float> *>(args);
+    const cpx<float> *i = static_cast<const cpx<float> *>(in);
+    cpx<float> *o = static_cast<cpx<float> *>(out);
+    if (sign < 0) hipLaunchKernelGGL((ct3_pair_kernel<-1, BIG>), dim3(grid), dim3(1024), 0, s, i, o, a);
+    else hipLaunchKernelGGL((ct3_pair_kernel<1, BIG>), dim3(grid), dim3(1024), 0, s, i, o, a);
+}
+
 #ifdef TOPE_AXIS_BIG
+#define TOPE_LAUNCH_CT3P &launch_ct3p_t<true>
 #define TOPE_LAUNCH_CT3(R) &launch_ct3_t<R, true>
 #define TOPE_LAUNCH(R, N, E, TB, CF, SP, DF) &launch_axis_t<R, N, E, TB, CF, SP, true, DF>
 #define TOPE_LAUNCH_DIT2(R, N, E, TB, SP) &launch_dit2_t<R, N, E, TB, SP, true>
 #else
+#define TOPE_LAUNCH_CT3P &launch_ct3p_t<false>
 #define TOPE_LAUNCH_CT3(R) &launch_ct3_t<R, false>
 #define TOPE_LAUNCH(R, N, E, TB, CF, SP, DF) &launch_axis_t<R, N, E, TB, CF, SP, false, DF>
 #define TOPE_LAUNCH_DIT2(R, N, E, TB, SP) &launch_dit2_t<R, N, E, TB, SP, false>
@@ -113,6 +127,11 @@ constexpr int prec_of() { return sizeof(R) == 8 ? PREC_F64 : PREC_F32; }
 #define VARC(R)                                                                                               \
     AxisVariant{prec_of<R>(), 4096, 16, 4, true, true, false, false, true, 1024, CT3_TW, Ct3Lds<R, true>::BYTES, \
                 TOPE_LAUNCH_CT3(R), "axis_" TOPE_STR(R) "_n4096_e16_tb4_col_ct3"}
+
+// Complex64 three-step column tile (4096 points, 8 columns as 4 pairs).
+#define VARCP                                                                                                 \
+    AxisVariant{PREC_F32, 4096, 16, 8, true, true, false, false, true, 1024, CT3_TW, Ct3PairLds::BYTES,       \
+                TOPE_LAUNCH_CT3P, "axis_float_n4096_e16_tb8_col_ct3"}
 
 // Even/odd-split column tiles for the long strided axes.
 #define DIT2COLS(R) VARH(R, 4096, 8, 4, true), VARH(R, 4096, 16, 4, true), VARH(R, 4096, 16, 4, false), \
@@ -157,7 +176,7 @@ constexpr int prec_of() { return sizeof(R) == 8 ? PREC_F64 : PREC_F32; }
 static const AxisVariant kVariants[] = {ROWS(double), COLS(double), DIF2ROWS(double), ROWS(float), COLS(float),
                                         DIT2COLS(double), VARH(float, 4096, 16, 8, true),
                                         VARH(float, 4096, 16, 8, false), VAR(float, 4096, 32, 8, true, true),
-                                        VARC(double)};
+                                        VARC(double), VARCP};
 
 // The same list is compiled twice: normally (32-bit buffer offsets) and with
 // TOPE_AXIS_BIG (64-bit pointers) for blocks spanning more than 2^31 bytes.
