@@ -268,7 +268,11 @@ void tope3DExecEx(struct topeFFT *f, struct topePlan3D *t, void *host, int dir);
  * element type and layout, interleaved complex) in place, asynchronously on
  * the framework's stream, without host copies; plan_impl is the plan
  * struct's impl field.  Not for real-input plans (topeRealExecPtr) or
- * packed uneven batches (topeManyExecPtr).  The ExecPtr calls enqueue the
+ * packed uneven batches (topeManyExecPtr).  Complex128 arrays must be
+ * 16-byte aligned (the element size); so must complex64 arrays of plans
+ * whose strided 4096-point axes move two columns as one 16-byte element
+ * (TOPE_ERROR_INVALID_ARG otherwise, before anything is enqueued; device
+ * allocations and torch tensors at offset 0 are).  The ExecPtr calls enqueue the
  * plan's passes on the caller's arrays and return; the plan can take the
  * next call at once (its own arrays are left untouched).  One host thread at
  * a time per framework and its plans. */
