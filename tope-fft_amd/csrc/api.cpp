@@ -225,7 +225,7 @@ const Preferred kPreferred[] = {
 // tile, which need tables of their own), for callers that provide no tables.
 // strided_io: the line elements are strided in the input and the output.
 // pair_ok: the complex64 three-step tile's conditions hold (adjacent
-// columns, an even column count, even strides, no post-twiddle).
+// columns, an even column count, even strides).
 const AxisVariant *select_variant(int prec, int n, bool cfast, long long lines, bool dif2 = false,
                                   long long outer = 1, bool plain_only = false, bool strided_io = false,
                                   bool pair_ok = false)
@@ -417,7 +417,7 @@ class Planner {
         // the complex64 three-step tile moves two adjacent columns as one
         // 16-byte element: every pair must start 16-byte aligned
         auto even = [](long long x) { return (x & 1) == 0; };
-        const bool pair_ok = !pt_lo && !dif2 && gi.cs == 1 && go.cs == 1 && even(gi.C) && even(gi.is) &&
+        const bool pair_ok = !dif2 && gi.cs == 1 && go.cs == 1 && even(gi.C) && even(gi.is) &&
                              even(go.is) && (gi.O == 1 || (even(gi.os) && even(go.os))) &&
                              (reps == 1 || (even(rep_in) && even(rep_out)));
         const AxisVariant *v = select_variant(P->prec, n, cfast, dif2 ? 2 * gi.C : gi.C, dif2, gi.O * reps, false,

@@ -198,12 +198,20 @@ __global__ void __launch_bounds__(1024) ct3_kernel(const cpx<R> *in, cpx<R> *out
 // exchanges are those of ct3_kernel, run once per column of the pair, with
 // the twiddles shared; each exchange moves whole complex values (8-byte LDS
 // accesses, the FP64 tile's bank pattern) in one round per column.
-// Requires unit column stride, C a multiple of 8 (whole pairs and tiles),
-// even row and plane strides (16-byte aligned pairs) and no post-twiddle.
+// Requires unit column stride, an even column count (whole pairs) and even
+// row and plane strides (16-byte aligned pairs).
+//
+// Four-step post-twiddle (POST) of column c (base b_c): output k = g + 16 w
+// + 256 e takes W^{b_c g} W^{16 b_c w} W^{256 b_c e}, three table values
+// per output (FP32: a fresh product instead of a chain of 16), read into LDS
+// at the tile's start: rows of 16 per column for g, w and e.
+constexpr int CT3P_PT = 3 * 8 * 16;
+
+template <bool POST>
 struct Ct3PairLds {
     static constexpr int WR = 16 * 68;  // complex slots per wave region
     static constexpr int XB = 16 * WR * (int)sizeof(cpx<float>);
-    static constexpr int BYTES = XB + CT3_TW * (int)sizeof(cpx<float>);
+    static constexpr int BYTES = XB + (CT3_TW + (POST ? CT3P_PT : 0)) * (int)sizeof(cpx<float>);
 };
 
 // One row of a column pair as four dwords (re, im of column 2t, then 2t + 1).
@@ -249,15 +257,25 @@ __device__ __forceinline__ void ct3_pair_cross(cpx<float> (&v)[16], cpx<float> *
     for (int k = 0; k < 16; ++k) mine[k * 64 + lane] = v[k];
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < 16; ++r) v[r] = xl[r * Ct3PairLds::WR + w * 64 + lane];
+    for (int r = 0; r < 16; ++r) v[r] = xl[r * Ct3PairLds<false>::WR + w * 64 + lane];
 }
 
-template <int SIGN, bool BIG>
+// Column cc's post-twiddle applied to its 16 outputs k = g + 16 w + 256 e.
+template <int SIGN>
+__device__ __forceinline__ void ct3_pair_post(cpx<float> (&v)[16], const cpx<float> *ptl, int cc, int g, int w)
+{
+    const cpx<float> s = cmul(ptl[16 * cc + g], ptl[128 + 16 * cc + w]);
+    v[0] = twmul<SIGN>(v[0], s);
+#pragma unroll
+    for (int e = 1; e < 16; ++e) v[e] = twmul<SIGN>(v[e], cmul(s, ptl[256 + 16 * cc + e]));
+}
+
+template <int SIGN, bool POST, bool BIG>
 __global__ void __launch_bounds__(1024) ct3_pair_kernel(const cpx<float> *in, cpx<float> *out,
                                                         const AxisArgs<float> a)
 {
     using R = float;
-    using L = Ct3PairLds;
+    using L = Ct3PairLds<POST>;
     constexpr int TB = 8;
     constexpr int CB = (int)sizeof(cpx<R>);
     __shared__ __align__(16) unsigned char lds[L::BYTES];
@@ -275,6 +293,17 @@ __global__ void __launch_bounds__(1024) ct3_pair_kernel(const cpx<float> *in, cp
     const int c0 = tile * TB;
     const int c = c0 + 2 * t;
     const bool valid = c < a.C;
+    // the post-twiddle factors of the tile's columns, read while the data
+    // loads are in flight
+    cpx<R> *ptl = ltw + CT3_TW;
+    if constexpr (POST) {
+        if (tid < CT3P_PT) {
+            const int kind = tid >> 7, cc = (tid >> 4) & 7, j = tid & 15;
+            const long long bt = o * a.pt_o + (long long)(c0 + cc) * a.pt_c;
+            const long long x = bt * (kind == 0 ? j : kind == 1 ? 16 * j : 256 * j);
+            ptl[tid] = pt_lookup(x, a.pt_lo, a.pt_hi, a.pt_n, a.pt_shift);
+        }
+    }
 
     // step A inputs: rows 256 e + 16 g + w of columns c, c + 1
     cpx<R> v0[16], v1[16];
@@ -339,6 +368,10 @@ __global__ void __launch_bounds__(1024) ct3_pair_kernel(const cpx<float> *in, cp
     dft16<SIGN>(v1);
 
     if (!valid) return;
+    if constexpr (POST) {
+        ct3_pair_post<SIGN>(v0, ptl, 2 * t, g, w);
+        ct3_pair_post<SIGN>(v1, ptl, 2 * t + 1, g, w);
+    }
     if (a.scale != R(1)) {
 #pragma unroll
         for (int e = 0; e < 16; ++e) {

@@ -85,8 +85,13 @@ static void launch_ct3p_t(const void *in, void *out, const void *args, unsigned 
     const AxisArgs<float> &a = *static_cast<const AxisArgs<float> *>(args);
     const cpx<float> *i = static_cast<const cpx<float> *>(in);
     cpx<float> *o = static_cast<cpx<float> *>(out);
-    if (sign < 0) hipLaunchKernelGGL((ct3_pair_kernel<-1, BIG>), dim3(grid), dim3(1024), 0, s, i, o, a);
-    else hipLaunchKernelGGL((ct3_pair_kernel<1, BIG>), dim3(grid), dim3(1024), 0, s, i, o, a);
+    if (a.pt_n) {
+        if (sign < 0) hipLaunchKernelGGL((ct3_pair_kernel<-1, true, BIG>), dim3(grid), dim3(1024), 0, s, i, o, a);
+        else hipLaunchKernelGGL((ct3_pair_kernel<1, true, BIG>), dim3(grid), dim3(1024), 0, s, i, o, a);
+    } else {
+        if (sign < 0) hipLaunchKernelGGL((ct3_pair_kernel<-1, false, BIG>), dim3(grid), dim3(1024), 0, s, i, o, a);
+        else hipLaunchKernelGGL((ct3_pair_kernel<1, false, BIG>), dim3(grid), dim3(1024), 0, s, i, o, a);
+    }
 }
 
 #ifdef TOPE_AXIS_BIG
@@ -130,7 +135,7 @@ constexpr int prec_of() { return sizeof(R) == 8 ? PREC_F64 : PREC_F32; }
 
 // Complex64 three-step column tile (4096 points, 8 columns as 4 pairs).
 #define VARCP                                                                                                 \
-    AxisVariant{PREC_F32, 4096, 16, 8, true, true, false, false, true, 1024, CT3_TW, Ct3PairLds::BYTES,       \
+    AxisVariant{PREC_F32, 4096, 16, 8, true, true, false, false, true, 1024, CT3_TW, Ct3PairLds<true>::BYTES,       \
                 TOPE_LAUNCH_CT3P, "axis_float_n4096_e16_tb8_col_ct3"}
 
 // Even/odd-split column tiles for the long strided axes.
