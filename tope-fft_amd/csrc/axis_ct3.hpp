@@ -94,6 +94,12 @@ __global__ void __launch_bounds__(1024) ct3_kernel(const cpx<R> *in, cpx<R> *out
         }
     }
 
+    // step B's twiddles W_256^{m g}, issued ahead of the data loads (read
+    // after the first exchange, they waited behind the streaming traffic)
+    cpx<R> twb[16];
+#pragma unroll
+    for (int m = 1; m < 16; m <<= 1) twb[m] = a.tw[m * g];
+
     // step A inputs: rows 256 e + 16 g + w of column c
     cpx<R> v[16];
     if constexpr (!BIG) {
@@ -138,12 +144,7 @@ __global__ void __launch_bounds__(1024) ct3_kernel(const cpx<R> *in, cpx<R> *out
 
     // step B: register n1 times W_256^{n1 k0} (k0 = g), DFT over n1 -> register k1
     // (the W_256 rows come from the global table: no barrier before this step)
-    {
-        cpx<R> tw[16];
-#pragma unroll
-        for (int m = 1; m < 16; m <<= 1) tw[m] = a.tw[m * g];
-        apply_twiddles<SIGN>(v, tw);
-    }
+    apply_twiddles<SIGN>(v, twb);
     dft16<SIGN>(v);
 
     // cross-wave exchange: register k1 -> wave k1, wave n0 -> register n0;
@@ -305,6 +306,11 @@ __global__ void __launch_bounds__(1024) ct3_pair_kernel(const cpx<float> *in, cp
         }
     }
 
+    // step B's twiddles, issued ahead of the data loads
+    cpx<R> twb[16];
+#pragma unroll
+    for (int m = 1; m < 16; m <<= 1) twb[m] = a.tw[m * g];
+
     // step A inputs: rows 256 e + 16 g + w of columns c, c + 1
     cpx<R> v0[16], v1[16];
     if (valid) {
@@ -339,14 +345,9 @@ __global__ void __launch_bounds__(1024) ct3_pair_kernel(const cpx<float> *in, cp
     ct3_pair_in_wave(v1, mine, g, t);
 
     // step B: register n1 times W_256^{n1 k0} (k0 = g), DFT over n1 -> register k1
-    {
-        cpx<R> tw[16];
+    apply_twiddles<SIGN>(v0, twb);  // forms every twb[m]
 #pragma unroll
-        for (int m = 1; m < 16; m <<= 1) tw[m] = a.tw[m * g];
-        apply_twiddles<SIGN>(v0, tw);  // forms every tw[m]
-#pragma unroll
-        for (int m = 1; m < 16; ++m) v1[m] = twmul<SIGN>(v1[m], tw[m]);
-    }
+    for (int m = 1; m < 16; ++m) v1[m] = twmul<SIGN>(v1[m], twb[m]);
     dft16<SIGN>(v0);
     dft16<SIGN>(v1);
 
