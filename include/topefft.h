@@ -268,11 +268,12 @@ void tope3DExecEx(struct topeFFT *f, struct topePlan3D *t, void *host, int dir);
  * element type and layout, interleaved complex) in place, asynchronously on
  * the framework's stream, without host copies; plan_impl is the plan
  * struct's impl field.  Not for real-input plans (topeRealExecPtr) or
- * packed uneven batches (topeManyExecPtr).  Complex128 arrays must be
- * 16-byte aligned (the element size); so must complex64 arrays of plans
- * whose strided 4096-point axes move two columns as one 16-byte element
- * (TOPE_ERROR_INVALID_ARG otherwise, before anything is enqueued; device
- * allocations and torch tensors at offset 0 are).  The ExecPtr calls enqueue the
+ * packed uneven batches (topeManyExecPtr).  The array must be aligned to
+ * its element size (16 bytes complex128, 8 bytes complex64), and to 16
+ * bytes for complex64 plans whose strided 4096-point axes move two columns
+ * as one 16-byte element; otherwise TOPE_ERROR_INVALID_ARG, before anything
+ * is enqueued (device allocations and tensors at offset 0 are aligned).
+ * The ExecPtr calls enqueue the
  * plan's passes on the caller's arrays and return; the plan can take the
  * next call at once (its own arrays are left untouched).  One host thread at
  * a time per framework and its plans. */
@@ -319,9 +320,10 @@ void topeRealExecEx(struct topeFFT *f, struct topePlanReal *t, void *host, int d
  * (realSize bytes), `spectrum` dataSize bytes (rows of x/2+1 complex values
  * at a pitch of `pitch` elements).  Asynchronous on the framework's stream, no host copies.
  * FORWARD writes `spectrum` and may overwrite `real`; INVERSE writes `real`
- * and may overwrite `spectrum`.  Plans with y or z above 4096 use the real
- * array's buffer as dataSize bytes of scratch and are refused
- * (TOPE_ERROR_UNSUPPORTED). */
+ * and may overwrite `spectrum`.  Both arrays aligned to the complex element
+ * size (16 bytes FP64, 8 bytes FP32; TOPE_ERROR_INVALID_ARG otherwise).
+ * Plans with y or z above 4096 use the real array's buffer as dataSize
+ * bytes of scratch and are refused (TOPE_ERROR_UNSUPPORTED). */
 void topeRealExecPtr(struct topeFFT *f, struct topePlanReal *t, void *real, void *spectrum, int dir);
 void topeRealDestroy(struct topeFFT *f, struct topePlanReal *t);
 
@@ -368,7 +370,8 @@ void topeManyExecDevice(struct topeFFT *f, struct topePlanMany *t, int dir);
 /* topeManyExecPtr: the plan's transforms on a caller-owned device array of
  * the same packed layout (dataSize bytes), in place, asynchronously on the
  * plan's stream; the graph methods run their launches without the graph
- * (the graphs hold the plan's own array). */
+ * (the graphs hold the plan's own array).  The array aligned to the element
+ * size (TOPE_ERROR_INVALID_ARG otherwise). */
 void topeManyExecPtr(struct topeFFT *f, struct topePlanMany *t, void *data, int dir);
 /* topeManyTimeExec: mean device time in milliseconds of one exec of the
  * plan's current method, from HIP events around `reps` back-to-back execs

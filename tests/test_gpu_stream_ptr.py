@@ -98,6 +98,14 @@ def test_exec_ptr_errors():
     p = topefft.Plan(fw, (16, 16), np.zeros(256, complex))
     with pytest.raises(topefft.TopeError):
         p.exec_ptr(0, topefft.FORWARD)
+    fw.synchronize()
+    # an array not aligned to the 16-byte element is refused before anything
+    # is enqueued (the plan's own array, offset by 8 bytes, stays zero)
+    with pytest.raises(topefft.TopeError) as e:
+        p.exec_ptr(p.t.data + 8, topefft.FORWARD)
+    assert e.value.code == topefft.TOPE_ERROR_INVALID_ARG
+    fw.synchronize()
+    assert not np.any(p.download())
     p.destroy()
     fw.destroy()
 
@@ -178,9 +186,19 @@ def test_real_and_many_exec_ptr_errors():
         p.exec_ptr(0, p.t.data, topefft.FORWARD)
     with pytest.raises(topefft.TopeError):
         p.exec_ptr(p.t.real, p.t.data, 7)
+    # arrays not aligned to the complex element are refused
+    with pytest.raises(topefft.TopeError) as e:
+        p.exec_ptr(p.t.real + 8, p.t.data, topefft.FORWARD)
+    assert e.value.code == topefft.TOPE_ERROR_INVALID_ARG
+    with pytest.raises(topefft.TopeError) as e:
+        p.exec_ptr(p.t.real, p.t.data + 8, topefft.FORWARD)
+    assert e.value.code == topefft.TOPE_ERROR_INVALID_ARG
     q = topefft.ManyPlan(fw, np.array([16, 32], np.int32), np.zeros(48, complex))
     with pytest.raises(topefft.TopeError):
         q.exec_ptr(0, topefft.FORWARD)
+    with pytest.raises(topefft.TopeError) as e:
+        q.exec_ptr(q.t.data + 8, topefft.FORWARD)
+    assert e.value.code == topefft.TOPE_ERROR_INVALID_ARG
     q.destroy()
     p.destroy()
     fw.destroy()

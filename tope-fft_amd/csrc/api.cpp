@@ -2089,7 +2089,10 @@ void topeRealExecPtr(struct topeFFT *f, struct topePlanReal *t, void *real, void
         f->error = TOPE_ERROR_NOT_INITIALIZED;
         return;
     }
-    if (!real || !spectrum || (dir != FORWARD && dir != INVERSE)) {
+    // the real array is read as complex elements: both arrays aligned to the
+    // complex element size
+    if (!real || !spectrum || (dir != FORWARD && dir != INVERSE) ||
+        reinterpret_cast<uintptr_t>(real) % P->elem || reinterpret_cast<uintptr_t>(spectrum) % P->elem) {
         f->error = TOPE_ERROR_INVALID_ARG;
         return;
     }
@@ -2160,8 +2163,10 @@ void topePlanExecPtr(struct topeFFT *f, void *plan_impl, void *data, int dir)
         f->error = !data ? TOPE_ERROR_INVALID_ARG : TOPE_ERROR_UNSUPPORTED;
         return;
     }
-    if (P->pair_tiles && (reinterpret_cast<uintptr_t>(data) & 15)) {
-        // complex64 column pairs are moved as 16-byte elements
+    // elements are loaded whole: the array aligned to the element size, or
+    // to 16 bytes where complex64 column pairs move as one element
+    const uintptr_t align = P->pair_tiles ? 16 : (uintptr_t)P->elem;
+    if (reinterpret_cast<uintptr_t>(data) % align) {
         f->error = TOPE_ERROR_INVALID_ARG;
         return;
     }

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -554,8 +555,8 @@ void topeManyExecPtr(struct topeFFT *f, struct topePlanMany *t, void *data, int 
         if (f) f->error = TOPE_ERROR_INVALID_ARG;
         return;
     }
-    if (!data || (dir != FORWARD && dir != INVERSE)) {
-        f->error = TOPE_ERROR_INVALID_ARG;
+    if (!data || (dir != FORWARD && dir != INVERSE) || reinterpret_cast<uintptr_t>(data) % M->elem) {
+        f->error = TOPE_ERROR_INVALID_ARG;  // elements are loaded whole
         return;
     }
     // the graphs hold the plan's own array: the graph methods run their
