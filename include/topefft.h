@@ -410,7 +410,8 @@ long long topePlanTwiddles(struct topeFFT *f, void *plan_impl, int axis, double 
  * count * n <= TOPE_NOALLOC_MAX_ELEMENTS, else f->error =
  * TOPE_ERROR_SCRATCH_LIMIT.  The static array is shared by every call on a
  * GPU: calls that overlap on one GPU (other streams, other host threads)
- * race.  FORWARD unnormalised, INVERSE scaled by 1/n.  Asynchronous. */
+ * race.  FORWARD unnormalised, INVERSE scaled by 1/n.  Asynchronous.
+ * `data` 16-byte aligned (TOPE_ERROR_INVALID_ARG otherwise). */
 #define TOPE_NOALLOC_MAX_ELEMENTS (1 << 21)
 void topeExec1DNoAlloc(struct topeFFT *f, void *data, int n, int count, int dir);
 

@@ -93,7 +93,13 @@ def test_noalloc_scratch_limit(torch):
     fw.f.error = 0
     torch.cuda.synchronize()
     assert not bool(torch.any(t))
-    # a good call after the refused one
+    # an array not aligned to the 16-byte element: refused, nothing launched
+    fw.lib.topeExec1DNoAlloc(C.byref(fw.f), C.c_void_p(t.data_ptr() + 8), 1 << 13, 1, topefft.FORWARD)
+    assert fw.f.error == topefft.TOPE_ERROR_INVALID_ARG
+    fw.f.error = 0
+    torch.cuda.synchronize()
+    assert not bool(torch.any(t))
+    # a good call after the refused ones
     m = 1 << 13
     y = _data(m, 4, 12)
     got = _noalloc(torch, fw, y, m, 4, topefft.INVERSE)

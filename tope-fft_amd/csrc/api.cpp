@@ -2218,8 +2218,8 @@ void topeExec1DNoAlloc(struct topeFFT *f, void *data, int n, int count, int dir)
         if (f) f->error = TOPE_ERROR_NOT_INITIALIZED;
         return;
     }
-    if (!data || (dir != FORWARD && dir != INVERSE)) {
-        f->error = TOPE_ERROR_INVALID_ARG;
+    if (!data || (dir != FORWARD && dir != INVERSE) || reinterpret_cast<uintptr_t>(data) % 16) {
+        f->error = TOPE_ERROR_INVALID_ARG;  // complex128 elements are loaded whole
         return;
     }
     f->error = noalloc_exec_1d(f->device, data, n, count, dir == INVERSE, fw->stream);
