@@ -130,3 +130,29 @@ def test_4096_complex64_pair_tile(field2d, mode, rows):
     assert err < 1e-6
     assert rt < 1e-6
     assert rel_l2(back, Xi) < 1e-6
+
+
+def test_4096_real_complex64_against_fp64():
+    # 2D 4096^2 R2C with complex64 storage: the row step packs the DC and
+    # Nyquist columns, the 4096-point columns run the paired-column tile,
+    # the last pass unpacks; the half spectrum against the FP64 real plan's
+    # (whose kernels are pinned to the oracle) and the C2R round trip
+    rng = np.random.default_rng(4096)
+    x = rng.uniform(-1.0, 1.0, 4096 * 4096)
+    fw = topefft.Framework()
+    p = topefft.RealPlan(fw, DIMS2, x)
+    X64 = p.forward()
+    # the next plan before this one is destroyed (the last destroy releases
+    # the framework)
+    q = topefft.RealPlan(fw, DIMS2, x.astype(np.float32), precision=topefft.PREC_F32)
+    p.destroy()
+    names = q.pass_names()
+    X32 = q.forward().astype(np.complex128)
+    back = q.inverse().astype(np.float64)
+    q.destroy()
+    fw.destroy()
+    assert names == ["real_rows_fused", "axis_float_n4096_e16_tb8_col_ct3", "real_pack_dc_nyquist"], names
+    err = rel_l2(X32, X64)
+    print(f"4096^2 real complex64 rel-L2 vs FP64 {err:.3e}")
+    assert err < 1e-6
+    assert rel_l2(back, x.astype(np.float32).astype(np.float64)) < 1e-6

@@ -188,3 +188,35 @@ def test_real_complex64_limits():
         topefft.RealPlan(fw, (16,), np.zeros(16, np.float32), precision=5)
     assert e.value.code == topefft.TOPE_ERROR_INVALID_ARG
     fw.destroy()
+
+
+def test_r2c_complex64_packed_pair_tile(oracle, monkeypatch):
+    # complex64 plans with column passes carry the DC and Nyquist columns
+    # packed as one complex column (an even column count), so 4096-point
+    # columns can run the paired-column tile: forced here onto 16 x 4096
+    # (8 packed columns), forward against the FP64 oracle and the round trip
+    monkeypatch.setenv("TOPEFFT_VARIANT_4096_col", "axis_float_n4096_e16_tb8_col_ct3")
+    dims = (16, 4096)
+    x = real_input(dims, 123).astype(np.float32)
+    fw = topefft.Framework()
+    p = topefft.RealPlan(fw, dims, x, precision=topefft.PREC_F32)
+    names = p.pass_names()
+    X = p.forward()
+    back = p.inverse()
+    p.destroy()
+    fw.destroy()
+    # 8-point rows: the row pass, then the half-spectrum step on its own
+    assert names[-3:] == ["real_half_spectrum", "axis_float_n4096_e16_tb8_col_ct3", "real_pack_dc_nyquist"], names
+    want = half(oracle.dft(x.astype(np.complex128), dims, topefft.FORWARD), dims)
+    assert rel_l2(X.astype(np.complex128), want) < 1e-6
+    assert rel_l2(back.astype(np.float64), x.astype(np.float64)) < 1e-6
+
+
+def test_r2c_fp64_keeps_nyquist_column():
+    # FP64 plans transform the m + 1 columns as they are (no pack pass)
+    fw = topefft.Framework()
+    p = topefft.RealPlan(fw, (64, 32), np.zeros(64 * 32))
+    names = p.pass_names()
+    p.destroy()
+    fw.destroy()
+    assert "real_pack_dc_nyquist" not in names, names

@@ -56,7 +56,8 @@ enum PassKind {
     PASS_MFMA_TILE = 3,
     PASS_REAL = 4,
     PASS_CHIRP = 5,
-    PASS_MIXED = 6
+    PASS_MIXED = 6,
+    PASS_RPACK = 7  // real plans: unpack (FORWARD) / pack (INVERSE) of the DC/Nyquist columns
 };
 
 struct Pass {
@@ -111,6 +112,7 @@ struct PlanImpl {
     bool real = false;
     long long real_m = 0, real_rows = 0, real_pitch = 0;
     double real_scale = 1.0;
+    bool real_pack = false;  // DC and Nyquist columns packed through the column passes
     size_t real_bytes = 0;
     std::vector<int> inv_order;
     bool real_scratch0 = false;  // a column pass uses buffer 0 (P->bytes) as scratch
@@ -1007,9 +1009,13 @@ int launch_pass(PlanImpl *P, size_t i, hipStream_t s, int dir)
         void *dst = P->buf[fwd ? ps.out_buf : ps.in_buf];
         hipError_t err =
             ps.real_stw ? launch_real_rows(P->prec, fwd ? 0 : 1, src, dst, ps.real_stw, ps.real_w, P->real_m, P->real_pitch,
-                                           P->real_rows, P->real_scale, s)
+                                           P->real_rows, P->real_scale, P->real_pack ? 1 : 0, s)
                         : launch_real_pass(P->prec, fwd ? 0 : 1, src, dst, ps.real_w, P->real_m, P->real_pitch, P->real_rows,
-                                           P->real_scale, s);
+                                           P->real_scale, P->real_pack ? 1 : 0, s);
+        if (err != hipSuccess) return map_err(err);
+    } else if (ps.kind == PASS_RPACK) {
+        hipError_t err = launch_real_pack(P->prec, dir == FORWARD ? 0 : 1, P->buf[ps.in_buf], P->real_m, P->real_pitch,
+                                          P->dims[1], P->dims[2], s);
         if (err != hipSuccess) return map_err(err);
     } else if (ps.kind == PASS_MFMA_TILE) {
         MfmaTileArgs m = ps.mt;
@@ -1373,12 +1379,22 @@ int build_real_plan(struct topeFFT *f, long long nx, long long ny, long long nz,
         P->passes.push_back(ps);
         group_end.push_back(P->passes.size());
         cur = 1;
-        // Y: the m + 1 columns kx <= m of every z plane in one pass; its
-        // last tile holds only the Nyquist column kx = m (its other lanes
-        // are masked and touch no memory) and is rotated to run first, so it
-        // overlaps the full tiles instead of trailing them.  Z likewise per
-        // y row.  Axes above 4096 run the four-step, with buffer 0 (the real
-        // array's, P->bytes long) as the scratch of its first pass.
+        // Y: the columns of every z plane in one pass, Z likewise per y row.
+        // FP64: the m + 1 columns kx <= m; the last tile holds only the
+        // Nyquist column kx = m (its other lanes are masked and touch no
+        // memory) and is rotated to run first, so it overlaps the full tiles
+        // instead of trailing them.  complex64: columns kx = 0 and m are
+        // real after the row step and travel packed as one complex column
+        // (X[0] + i X[m] in column 0), so the passes transform an even
+        // number of columns, m, which lets the 4096-point columns run the
+        // paired-column tile (2D 4096^2 complex64: 71.5 -> 62.0 us); a last
+        // pass unpacks them (INVERSE: packs first).  The packed form
+        // measured the same for FP64 (116.7 against 117.0 us at 4096^2: the
+        // column pass gains what the unpack pass costs).  Axes above 4096
+        // run the four-step, with buffer 0 (the real array's, P->bytes long)
+        // as the scratch of its first pass.
+        P->real_pack = (ny > 1 || nz > 1) && prec == PREC_F32;
+        const long long cols = P->real_pack ? m : m + 1;
         auto axis2 = [&](long long n, Geo g, int id) {
             if (n == 1) return (int)TOPE_SUCCESS;
             const size_t before = P->passes.size();
@@ -1386,15 +1402,23 @@ int build_real_plan(struct topeFFT *f, long long nx, long long ny, long long nz,
             if (r) return r;
             for (size_t i = before; i < P->passes.size(); ++i) {
                 Pass &ps = P->passes[i];
-                if (ps.kind == PASS_AXIS) ps.a.rot = ps.a32.rot = ps.a.tiles - 1;
+                if (ps.kind == PASS_AXIS && !P->real_pack) ps.a.rot = ps.a32.rot = ps.a.tiles - 1;
                 if (ps.in_buf == 0 || ps.out_buf == 0) P->real_scratch0 = true;
             }
             group_end.push_back(P->passes.size());
             return (int)TOPE_SUCCESS;
         };
-        rc = axis2(ny, Geo{nz, pitch * ny, m + 1, 1, pitch}, 1);
-        if (!rc) rc = axis2(nz, Geo{ny, pitch, m + 1, 1, pitch * ny}, 2);
+        rc = axis2(ny, Geo{nz, pitch * ny, cols, 1, pitch}, 1);
+        if (!rc) rc = axis2(nz, Geo{ny, pitch, cols, 1, pitch * ny}, 2);
         if (!rc && cur != 1) rc = TOPE_ERROR_UNSUPPORTED;
+        if (!rc && P->real_pack) {
+            Pass pk;
+            pk.kind = PASS_RPACK;
+            pk.in_buf = pk.out_buf = 1;
+            pk.name = "real_pack_dc_nyquist";
+            P->passes.push_back(pk);
+            group_end.push_back(P->passes.size());
+        }
     }
     if (rc) return fail(rc);
     for (size_t g = group_end.size(); g-- > 0;) {
@@ -2090,9 +2114,11 @@ void topeRealExecPtr(struct topeFFT *f, struct topePlanReal *t, void *real, void
         return;
     }
     // the real array is read as complex elements: both arrays aligned to the
-    // complex element size
+    // complex element size, the spectrum to 16 bytes where complex64 column
+    // pairs move as one element
+    const uintptr_t salign = P->pair_tiles ? 16 : (uintptr_t)P->elem;
     if (!real || !spectrum || (dir != FORWARD && dir != INVERSE) ||
-        reinterpret_cast<uintptr_t>(real) % P->elem || reinterpret_cast<uintptr_t>(spectrum) % P->elem) {
+        reinterpret_cast<uintptr_t>(real) % P->elem || reinterpret_cast<uintptr_t>(spectrum) % salign) {
         f->error = TOPE_ERROR_INVALID_ARG;
         return;
     }

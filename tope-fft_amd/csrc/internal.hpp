@@ -70,16 +70,23 @@ hipError_t launch_mfma_rows(int mode, int n, const void *in, void *out, const Mf
 // Real-input transforms (kernels_real.hip): dir 0 = R2C half-spectrum
 // post-processing (Z, row pitch m -> X, m + 1 values per row at row pitch
 // `pitch`), 1 = C2R pre-processing (X -> Z, scaled by s); w = W_n^k,
-// k = 0..m, n = 2m.
+// k = 0..m, n = 2m.  pack: column 0 carries X[0] + i X[m] (column m
+// unused) for plans whose column passes run on the packed form.
 hipError_t launch_real_pass(int prec, int dir, const void *in, void *out, const void *w, long long m, long long pitch,
-                            long long rows, double s, hipStream_t st);
+                            long long rows, double s, int pack, hipStream_t st);
 // The m-point row transform fused with that step (64 <= m <= 4096, powers of
 // two): dir 0 reads the real rows (pitch m complex) and writes the half
 // spectrum, dir 1 the reverse (scaled by s); stw is the stage twiddle table
 // of schedule (m, 16).
 bool real_rows_supported(long long m);
 hipError_t launch_real_rows(int prec, int dir, const void *in, void *out, const void *stw, const void *w, long long m,
-                            long long pitch, long long rows, double s, hipStream_t st);
+                            long long pitch, long long rows, double s, int pack, hipStream_t st);
+// Packed DC/Nyquist columns of the half spectrum X (rows K = ky + ny kz at
+// row pitch `pitch`): dir 0 splits column 0's FZ = F0 + i Fm into column 0
+// (F0) and column m (Fm) after the forward column passes; dir 1 packs
+// F0 + i Fm into column 0 before the inverse ones.
+hipError_t launch_real_pack(int prec, int dir, void *X, long long m, long long pitch, long long ny, long long nz,
+                            hipStream_t st);
 // Bluestein chirp-z passes for non-power-of-two axes (kernels_chirp.hip),
 // FP64: n the axis length, M the power-of-two convolution length; the lines
 // (o < O, c < C) of the axis sit at o os + c cs, elements is apart, and
