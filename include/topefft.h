@@ -321,7 +321,9 @@ void topeRealExecEx(struct topeFFT *f, struct topePlanReal *t, void *host, int d
  * at a pitch of `pitch` elements).  Asynchronous on the framework's stream, no host copies.
  * FORWARD writes `spectrum` and may overwrite `real`; INVERSE writes `real`
  * and may overwrite `spectrum`.  Both arrays aligned to the complex element
- * size (16 bytes FP64, 8 bytes FP32; TOPE_ERROR_INVALID_ARG otherwise).
+ * size (16 bytes FP64, 8 bytes FP32), the spectrum to 16 bytes for complex64
+ * plans whose 4096-point column passes move column pairs as 16-byte
+ * elements (TOPE_ERROR_INVALID_ARG otherwise).
  * Plans with y or z above 4096 use the real array's buffer as dataSize
  * bytes of scratch and are refused (TOPE_ERROR_UNSUPPORTED). */
 void topeRealExecPtr(struct topeFFT *f, struct topePlanReal *t, void *real, void *spectrum, int dir);
