@@ -185,3 +185,28 @@ def test_refuses_beyond_the_three_pass_limit():
                           tiny.ctypes.data_as(C.POINTER(C.c_double)))
     assert fw.f.error == topefft.TOPE_ERROR_INVALID_SIZE
     fw.destroy()
+
+
+def test_2d_pair_tile_64bit_addressing_complex64(oracle):
+    # complex64 65536 x 4096 (2 GiB): the 4096-point column pass spans more
+    # than 2^31 bytes per tile (rows 512 KiB apart), so it runs the 64-bit
+    # addressing instantiation of the paired-column three-step tile; the
+    # rows run the four-step; sampled bins against the oracle's direct DFT
+    # (complex64 storage: rel-L2 <= 1e-6) and the device round trip
+    dims = (65536, 4096)
+    n = dims[0] * dims[1]
+    x = seeded(n, 11).astype(np.complex64)
+    fw = topefft.Framework()
+    p = topefft.Plan(fw, dims, x, precision=topefft.PREC_F32)
+    names = p.pass_names()
+    X = p.exec(topefft.FORWARD)
+    back = p.exec(topefft.INVERSE)
+    p.destroy()
+    fw.destroy()
+    assert names[-1] == "axis_float_n4096_e16_tb8_col_ct3", names
+    bins = sample_bins(n, 110, 4)
+    want = oracle.dft_bins(x[: n].astype(np.complex128), (dims[0], dims[1], 1), bins, topefft.FORWARD)
+    err = rel_l2(X[bins].astype(np.complex128), want)
+    print(f"65536 x 4096 complex64 rel-L2 at sampled bins {err:.3e}")
+    assert err < 1e-6
+    assert rel_l2(back.astype(np.complex128), x.astype(np.complex128)) < 1e-6
