@@ -8,7 +8,8 @@ transforms are too long for the full naive DFT, so each is checked at
 sampled bins against the oracle's direct DFT (FP64: rel-L2 <= 1e-12;
 complex64 storage <= 1e-6), by Parseval and by the round trip on the
 device; the pass lists are asserted.  The 2^27-point FP64 array (2 GiB)
-takes the 64-bit addressing variants of the passes.
+takes the 64-bit addressing variants of the passes, and so do the column
+tiles of 40000 x 4096 FP64 and 65536 x 4096 complex64.
 """
 import ctypes as C
 
