@@ -59,6 +59,40 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Table values read at a block's start and used after its data loads are
+// issued travel as raw vectors: a complex local held across the loads was
+// kept by the compiler in an LDS slot of its own (16 KiB more LDS and a wait
+// for the read before the first data load).
+template <typename R>
+struct Ct3Raw {
+    typedef tope_u32x4 type;
+};
+template <>
+struct Ct3Raw<float> {
+    typedef tope_u32x2 type;
+};
+template <typename R>
+__device__ __forceinline__ typename Ct3Raw<R>::type raw_ld(const cpx<R> *p)
+{
+    return *reinterpret_cast<const typename Ct3Raw<R>::type *>(p);
+}
+template <typename R>
+__device__ __forceinline__ cpx<R> raw_cpx(typename Ct3Raw<R>::type u)
+{
+    return __builtin_bit_cast(cpx<R>, u);
+}
+
+// W_{pt_n}^x as the two table values whose product it is (pt_lookup's
+// operands): the reads are issued here, the product formed later.
+template <typename R>
+__device__ __forceinline__ void pt_fetch(long long x, const AxisArgs<R> &a, typename Ct3Raw<R>::type &lo,
+                                         typename Ct3Raw<R>::type &hi)
+{
+    x &= a.pt_n - 1;
+    lo = raw_ld(a.pt_lo + (x & ((1LL << a.pt_shift) - 1)));
+    hi = raw_ld(a.pt_hi + (x >> a.pt_shift));
+}
+
 template <typename R, int SIGN, bool POST, bool BIG>
 __global__ void __launch_bounds__(1024) ct3_kernel(const cpx<R> *in, cpx<R> *out, const AxisArgs<R> a)
 {
@@ -71,8 +105,19 @@ __global__ void __launch_bounds__(1024) ct3_kernel(const cpx<R> *in, cpx<R> *out
 
     const int tid = threadIdx.x;
     const int t = tid & 3, g = (tid >> 2) & 15, w = tid >> 6;
-    // step C's twiddle rows (read after the first workgroup barrier)
-    if (tid < CT3_TW) ltw[tid] = a.tw[tid];
+    // step C's twiddle rows (read after the first workgroup barrier).
+    // Without the post-twiddle: read into a register now (every thread, at a
+    // clamped index), stored to LDS once the data loads are issued -- a
+    // store right after the read waited for it before any data load went
+    // out (2D 4096^2 Y pass 105.0-106.3 -> 103.6-105.4 us).  With it the
+    // tile is at its register budget, and the held value made the inverse
+    // instantiation spill (1D 2^24 307 -> 314 us): stored at once there.
+    typename Ct3Raw<R>::type tabv{};
+    if constexpr (POST) {
+        if (tid < CT3_TW) ltw[tid] = a.tw[tid];
+    } else {
+        tabv = raw_ld(a.tw + (tid < CT3_TW ? tid : 0));
+    }
 
     const long long blk = xcd_block(a.xcd);
     const long long o = blk / a.tiles;
@@ -82,8 +127,8 @@ __global__ void __launch_bounds__(1024) ct3_kernel(const cpx<R> *in, cpx<R> *out
     const int c = c0 + t;
     const bool valid = c < a.C;
     // the post-twiddle factors of the tile's columns, read from the tables
-    // while the data loads are in flight (at the end of the tile, after the
-    // last barrier, table reads from L2 wait behind the streaming traffic)
+    // before the data loads (at the end of the tile, after the last barrier,
+    // table reads from L2 wait behind the streaming traffic)
     cpx<R> *ptl = ltw + CT3_TW;
     if constexpr (POST) {
         if (tid < CT3_PT) {
@@ -123,6 +168,12 @@ __global__ void __launch_bounds__(1024) ct3_kernel(const cpx<R> *in, cpx<R> *out
 #pragma unroll
             for (int e = 0; e < 16; ++e) v[e] = cpx<R>{0, 0};
         }
+    }
+    if constexpr (!POST) {
+        // keeps the compiler from joining this store to the read above
+        // (which would put the wait for the read before the data loads)
+        asm volatile("" ::: "memory");
+        if (tid < CT3_TW) ltw[tid] = raw_cpx<R>(tabv);
     }
 
     dft16<SIGN>(v);  // register k0
@@ -285,7 +336,8 @@ __global__ void __launch_bounds__(1024) ct3_pair_kernel(const cpx<float> *in, cp
 
     const int tid = threadIdx.x;
     const int t = tid & 3, g = (tid >> 2) & 15, w = tid >> 6;
-    if (tid < CT3_TW) ltw[tid] = a.tw[tid];
+    // step C's twiddle rows: read now, stored to LDS once the data loads are issued
+    const typename Ct3Raw<R>::type tabv = raw_ld(a.tw + (tid < CT3_TW ? tid : 0));  // as in ct3_kernel
 
     const long long blk = xcd_block(a.xcd);
     const long long o = blk / a.tiles;
@@ -294,16 +346,15 @@ __global__ void __launch_bounds__(1024) ct3_pair_kernel(const cpx<float> *in, cp
     const int c0 = tile * TB;
     const int c = c0 + 2 * t;
     const bool valid = c < a.C;
-    // the post-twiddle factors of the tile's columns, read while the data
-    // loads are in flight
+    // the post-twiddle factors of the tile's columns: read now, formed and
+    // stored to LDS once the data loads are issued
     cpx<R> *ptl = ltw + CT3_TW;
+    typename Ct3Raw<R>::type plo{}, phi{};
     if constexpr (POST) {
-        if (tid < CT3P_PT) {
-            const int kind = tid >> 7, cc = (tid >> 4) & 7, j = tid & 15;
-            const long long bt = o * a.pt_o + (long long)(c0 + cc) * a.pt_c;
-            const long long x = bt * (kind == 0 ? j : kind == 1 ? 16 * j : 256 * j);
-            ptl[tid] = pt_lookup(x, a.pt_lo, a.pt_hi, a.pt_n, a.pt_shift);
-        }
+        const int kind = tid >> 7, cc = (tid >> 4) & 7, j = tid & 15;
+        const long long bt = o * a.pt_o + (long long)(c0 + cc) * a.pt_c;
+        const long long x = tid >= CT3P_PT ? 0 : bt * (kind == 0 ? j : kind == 1 ? 16 * j : 256 * j);
+        pt_fetch(x, a, plo, phi);
     }
 
     // step B's twiddles, issued ahead of the data loads
@@ -334,6 +385,11 @@ __global__ void __launch_bounds__(1024) ct3_pair_kernel(const cpx<float> *in, cp
             v0[e] = cpx<R>{0, 0};
             v1[e] = cpx<R>{0, 0};
         }
+    }
+    asm volatile("" ::: "memory");  // as in ct3_kernel: the stores stay after the data loads
+    if (tid < CT3_TW) ltw[tid] = raw_cpx<R>(tabv);
+    if constexpr (POST) {
+        if (tid < CT3P_PT) ptl[tid] = cmul(raw_cpx<R>(plo), raw_cpx<R>(phi));
     }
 
     dft16<SIGN>(v0);  // register k0
