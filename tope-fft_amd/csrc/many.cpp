@@ -35,20 +35,19 @@ struct ManyImpl {
     std::vector<long long> offs;
     void *data = nullptr;
     std::vector<void *> allocs;
-    // persistent-kernel tables
-    ManyTask *d_tasks = nullptr;
-    int *d_order = nullptr;
-    long long *d_offs = nullptr;
+    // persistent-kernel tables: tasks of class L are seg0[L] .. seg1[L];
+    // the class's transforms sit at positions pos0[L] .. pos0[L] + cnt[L]
+    // of the class-grouped order, d_toffs[pos] the element offset of the
+    // transform at position pos, posof[i] the position of transform i
+    long long *d_toffs = nullptr;
     void *d_tw[16][2] = {};  // stage tables per class: [L][0] E = 16, [L][1] E = 8
-    int ntasks = 0;
     int seg0[16] = {}, seg1[16] = {};
+    int pos0[16] = {}, cnt[16] = {};
+    std::vector<int> posof;
     int sched[16] = {};
     int grid = 0;        // blocks of the 1024-thread kernel (classes of P > 256)
     int grid_small = 0;  // blocks of the 256-thread kernel (P <= 256)
     int method = TOPE_MANY_PERSISTENT;
-    // per-transform baseline: single-transform tasks over an identity order
-    ManyTask *d_single = nullptr;
-    int *d_ident = nullptr;
     // graphs per (method, dir)
     hipGraphExec_t gexec[4][2] = {};
     hipEvent_t e0 = nullptr, e1 = nullptr;  // timing only (autotuner): no system fence
@@ -99,38 +98,28 @@ void drop_graphs(ManyImpl *M)
 // their packed order, TB(class) transforms per task.
 hipError_t build_tasks(ManyImpl *M)
 {
-    std::vector<int> order;
-    std::vector<ManyTask> tasks;
-    for (int L = 0; L < 16; ++L) M->seg0[L] = M->seg1[L] = 0;
+    std::vector<long long> toffs;
+    M->posof.assign(M->count, 0);
+    int ntasks = 0;
+    for (int L = 0; L < 16; ++L) M->seg0[L] = M->seg1[L] = M->pos0[L] = M->cnt[L] = 0;
     for (int L = MANY_LMAX; L >= MANY_LMIN; --L) {
         const int tb = class_threads(M, L) / many_P(L, M->sched[L] != 0);
-        const int first = (int)order.size();
+        M->pos0[L] = (int)toffs.size();
         for (int i = 0; i < M->count; ++i)
-            if (M->sizes[i] == (1 << L)) order.push_back(i);
-        const int n_in = (int)order.size() - first;
-        M->seg0[L] = (int)tasks.size();
-        for (int k = 0; k < n_in; k += tb) {
-            ManyTask t;
-            t.cls = L;
-            t.first = first + k;
-            t.count = std::min(tb, n_in - k);
-            t.pad = 0;
-            tasks.push_back(t);
-        }
-        M->seg1[L] = (int)tasks.size();
+            if (M->sizes[i] == (1 << L)) {
+                M->posof[i] = (int)toffs.size();
+                toffs.push_back(M->offs[i]);
+            }
+        M->cnt[L] = (int)toffs.size() - M->pos0[L];
+        M->seg0[L] = ntasks;
+        ntasks += (M->cnt[L] + tb - 1) / tb;
+        M->seg1[L] = ntasks;
     }
-    M->ntasks = (int)tasks.size();
-    if (M->d_tasks) (void)hipFree(M->d_tasks);
-    if (M->d_order) (void)hipFree(M->d_order);
-    M->d_tasks = nullptr;
-    M->d_order = nullptr;
-    hipError_t err = hipMalloc(&M->d_tasks, sizeof(ManyTask) * std::max<size_t>(1, tasks.size()));
-    if (err == hipSuccess) err = hipMalloc(&M->d_order, sizeof(int) * std::max<size_t>(1, order.size()));
-    if (err == hipSuccess && !tasks.empty())
-        err = hipMemcpyAsync(M->d_tasks, tasks.data(), sizeof(ManyTask) * tasks.size(), hipMemcpyHostToDevice,
+    hipError_t err = hipSuccess;
+    if (!M->d_toffs) err = hipMalloc(&M->d_toffs, sizeof(long long) * std::max<size_t>(1, toffs.size()));
+    if (err == hipSuccess && !toffs.empty())
+        err = hipMemcpyAsync(M->d_toffs, toffs.data(), sizeof(long long) * toffs.size(), hipMemcpyHostToDevice,
                              M->stream);
-    if (err == hipSuccess && !order.empty())
-        err = hipMemcpyAsync(M->d_order, order.data(), sizeof(int) * order.size(), hipMemcpyHostToDevice, M->stream);
     if (err == hipSuccess) err = hipStreamSynchronize(M->stream);
     drop_graphs(M);
     return err;
@@ -142,17 +131,16 @@ template <typename R>
 ManyArgs<R> many_args(const ManyImpl *M, int threads, int only_class = -1)
 {
     ManyArgs<R> a{};
-    a.tasks = M->d_tasks;
-    a.ntasks = M->ntasks;
     for (int L = 0; L < 16; ++L) {
         const bool on = (only_class < 0 || only_class == L) && class_threads(M, L) == threads;
         a.seg0[L] = on ? M->seg0[L] : 0;
         a.seg1[L] = on ? M->seg1[L] : 0;
+        a.pos0[L] = M->pos0[L];
+        a.cnt[L] = M->cnt[L];
         a.sched[L] = M->sched[L];
         a.tw[L] = static_cast<const cpx<R> *>(M->d_tw[L][M->sched[L] ? 1 : 0]);
     }
-    a.order = M->d_order;
-    a.offs = M->d_offs;
+    a.toffs = M->d_toffs;
     a.elems = M->elems;
     return a;
 }
@@ -200,7 +188,7 @@ int launch_persistent(ManyImpl *M, int dir, hipStream_t s, int only_class = -1, 
 }
 
 // Baseline: one launch per transform -- the same device code on a
-// one-task list, one block, in place at the transform's offset.
+// one-transform class range, one block, in place at the transform's offset.
 int launch_per_transform(ManyImpl *M, int dir, hipStream_t s)
 {
     const int sign = dir == FORWARD ? -1 : 1;
@@ -210,19 +198,17 @@ int launch_per_transform(ManyImpl *M, int dir, hipStream_t s)
         hipError_t err;
         if (M->prec == PREC_F64) {
             ManyArgs<double> a = many_args<double>(M, nt, L);
-            a.tasks = M->d_single + i;
-            a.ntasks = 1;
-            a.order = M->d_ident;
             a.seg0[L] = 0;
             a.seg1[L] = 1;
+            a.pos0[L] = M->posof[i];
+            a.cnt[L] = 1;
             err = launch_many(PREC_F64, nt, M->data, M->data, &a, 1, sign, s);
         } else {
             ManyArgs<float> a = many_args<float>(M, nt, L);
-            a.tasks = M->d_single + i;
-            a.ntasks = 1;
-            a.order = M->d_ident;
             a.seg0[L] = 0;
             a.seg1[L] = 1;
+            a.pos0[L] = M->posof[i];
+            a.cnt[L] = 1;
             err = launch_many(PREC_F32, nt, M->data, M->data, &a, 1, sign, s);
         }
         if (err != hipSuccess) return map_err(err);
@@ -268,11 +254,7 @@ void free_many(ManyImpl *M)
     drop_graphs(M);
     for (void *p : M->allocs) (void)hipFree(p);
     if (M->data) (void)hipFree(M->data);
-    if (M->d_tasks) (void)hipFree(M->d_tasks);
-    if (M->d_order) (void)hipFree(M->d_order);
-    if (M->d_offs) (void)hipFree(M->d_offs);
-    if (M->d_single) (void)hipFree(M->d_single);
-    if (M->d_ident) (void)hipFree(M->d_ident);
+    if (M->d_toffs) (void)hipFree(M->d_toffs);
     if (M->e0) (void)hipEventDestroy(M->e0);
     if (M->e1) (void)hipEventDestroy(M->e1);
     if (M->x0) (void)hipEventDestroy(M->x0);
@@ -436,9 +418,6 @@ void topeManyPlanInit(struct topeFFT *f, struct topePlanMany *t, int count, cons
     if (err == hipSuccess) err = hipEventCreate(&p1);
     if (err == hipSuccess) err = hipEventRecord(p0, s);
     if (err == hipSuccess) err = hipMalloc(&M->data, (size_t)M->elems * M->elem);
-    if (err == hipSuccess) err = hipMalloc(&M->d_offs, sizeof(long long) * count);
-    if (err == hipSuccess)
-        err = hipMemcpyAsync(M->d_offs, M->offs.data(), sizeof(long long) * count, hipMemcpyHostToDevice, s);
     // twiddles for every class present, both schedules
     bool present[16] = {};
     for (int i = 0; i < count; ++i) present[ilog2i(sizes[i])] = true;
@@ -457,21 +436,7 @@ void topeManyPlanInit(struct topeFFT *f, struct topePlanMany *t, int count, cons
             err = gen_stage_twiddles(tw, M->prec, n, e, s);
         }
     }
-    if (err == hipSuccess) {
-        std::vector<ManyTask> single(count);
-        std::vector<int> ident(count);
-        for (int i = 0; i < count; ++i) {
-            single[i] = ManyTask{ilog2i(sizes[i]), i, 1, 0};
-            ident[i] = i;
-        }
-        err = hipMalloc(&M->d_single, sizeof(ManyTask) * count);
-        if (err == hipSuccess) err = hipMalloc(&M->d_ident, sizeof(int) * count);
-        if (err == hipSuccess)
-            err = hipMemcpyAsync(M->d_single, single.data(), sizeof(ManyTask) * count, hipMemcpyHostToDevice, s);
-        if (err == hipSuccess)
-            err = hipMemcpyAsync(M->d_ident, ident.data(), sizeof(int) * count, hipMemcpyHostToDevice, s);
-        if (err == hipSuccess) err = hipStreamSynchronize(s);
-    }
+    if (err == hipSuccess) err = hipStreamSynchronize(s);
     tope_ulong pre = 0, mem = 0;
     if (err == hipSuccess) err = hipEventRecord(p1, s);
     if (err == hipSuccess) err = hipEventSynchronize(p1);

@@ -24,33 +24,46 @@ constexpr int MANY_THREADS = 1024;
 constexpr int MANY_SMALL_THREADS = 256;
 constexpr int MANY_LMIN = 4, MANY_LMAX = 14;
 
+// One task: `count` transforms (<= TB of the class schedule) from position
+// `first` of the class-grouped order.
 struct ManyTask {
-    int cls;    // log2 n
-    int first;  // index into order[]
-    int count;  // transforms in this task (<= TB of the class schedule)
-    int pad;
+    int first;
+    int count;
 };
 
+// Task j of class L (seg0[L] <= j < seg1[L]) covers the positions
+// pos0[L] + k TB .. of the class-grouped order, k = j - seg0[L]: the task's
+// range is arithmetic on kernel arguments, and a transform's element offset
+// is one read of toffs (offsets in that order), so a block starts its data
+// loads one dependent read after taking a task rather than three (task
+// record, order entry, offset).
 template <typename R>
 struct ManyArgs {
-    const ManyTask *tasks;
-    int ntasks;
-    int seg0[16], seg1[16];  // tasks of class L are tasks[seg0[L] .. seg1[L])
-    const int *order;       // transform ids grouped by class
-    const long long *offs;  // element offset of each transform in the packed buffer
-    const cpx<R> *tw[16];   // stage twiddles of each class (layout of its schedule)
-    int sched[16];          // per class: 0 -> E = 16, 1 -> E = 8
-    long long elems;        // packed buffer length (elements)
+    int seg0[16], seg1[16];  // tasks of class L: seg0[L] .. seg1[L]
+    int pos0[16], cnt[16];   // class L: first position in the order, transforms
+    const long long *toffs;  // element offset of the transform at each position of the order
+    const cpx<R> *tw[16];    // stage twiddles of each class (layout of its schedule)
+    int sched[16];           // per class: 0 -> E = 16, 1 -> E = 8
+    long long elems;         // packed buffer length (elements)
 };
 
+template <int L, int TB, typename A>
+__device__ __forceinline__ ManyTask many_task(const A &a, int k)
+{
+    ManyTask tk;
+    tk.first = a.pos0[L] + k * TB;
+    tk.count = min(TB, a.cnt[L] - k * TB);
+    return tk;
+}
+
 template <typename R, int N, int E, int SIGN, int NT>
-__device__ __forceinline__ void many_rows(const cpx<R> *in, cpx<R> *out, const ManyArgs<R> &a, const ManyTask &tk,
-                                          void *lds)
+__device__ __forceinline__ void many_rows(const cpx<R> *in, cpx<R> *out, const ManyArgs<R> &a, int k, void *lds)
 {
     constexpr int P = N / E;
     constexpr int TB = NT / P;
     using SC = Sched<N, E>;
     constexpr int CB = (int)sizeof(cpx<R>);
+    const ManyTask tk = many_task<SC::LN, TB>(a, k);  // task k of the class
     // Opaque to the optimiser: nothing lane-dependent (LDS bases, twiddle
     // and data offsets) is hoisted out of the task loop, where it would stay
     // live across every class and spill under the 128-VGPR cap of a
@@ -59,7 +72,7 @@ __device__ __forceinline__ void many_rows(const cpx<R> *in, cpx<R> *out, const M
     asm volatile("" : "+v"(tid));
     const int t = tid / P, p = tid % P;
     const bool valid = t < tk.count;
-    const long long off = valid ? a.offs[a.order[tk.first + t]] : 0;
+    const long long off = valid ? a.toffs[tk.first + t] : 0;
     const __amdgpu_buffer_rsrc_t rsrc = make_rsrc(in, a.elems * CB);
     const __amdgpu_buffer_rsrc_t wsrc = make_rsrc(out, a.elems * CB);
     const int voff = (int)((off + p) * CB);
@@ -82,17 +95,17 @@ constexpr int many_lds_bytes() { return NT / 16 * 17 * 16 * (int)sizeof(R); }
 // Dispatch one task of class L to its schedule (E = 8 only where n/8 threads
 // fit the block).
 template <typename R, int L, int SIGN, int NT>
-__device__ __forceinline__ void many_class(const cpx<R> *in, cpx<R> *out, const ManyArgs<R> &a, const ManyTask &tk,
-                                           void *lds, bool e8)
+__device__ __forceinline__ void many_class(const cpx<R> *in, cpx<R> *out, const ManyArgs<R> &a, int k, void *lds,
+                                           bool e8)
 {
     constexpr int N = 1 << L;
     if constexpr (N >= 8 && N / 8 <= NT) {
         if (e8) {
-            many_rows<R, N, 8, SIGN, NT>(in, out, a, tk, lds);
+            many_rows<R, N, 8, SIGN, NT>(in, out, a, k, lds);
             return;
         }
     }
-    if constexpr ((N < 16 ? 1 : N / 16) <= NT) many_rows<R, N, (N < 16 ? N : 16), SIGN, NT>(in, out, a, tk, lds);
+    if constexpr ((N < 16 ? 1 : N / 16) <= NT) many_rows<R, N, (N < 16 ? N : 16), SIGN, NT>(in, out, a, k, lds);
 }
 
 // Tasks of one class: this block takes global task indices b, b + G, ...
@@ -108,12 +121,7 @@ __device__ __forceinline__ void many_segment(const cpx<R> *in, cpx<R> *out, cons
     int task = s0 + ((b - s0) % G + G) % G;
     const bool e8 = a.sched[L] != 0;
     for (; task < s1; task += G) {
-        ManyTask tk = a.tasks[task];
-        // the task is block-uniform: keep its fields in scalar registers
-        tk.cls = L;
-        tk.first = __builtin_amdgcn_readfirstlane(tk.first);
-        tk.count = __builtin_amdgcn_readfirstlane(tk.count);
-        many_class<R, L, SIGN, NT>(in, out, a, tk, lds, e8);
+        many_class<R, L, SIGN, NT>(in, out, a, task - s0, lds, e8);
         __syncthreads();  // the next task reuses the LDS
     }
 }
