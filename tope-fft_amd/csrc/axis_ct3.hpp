@@ -59,34 +59,11 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Table values read at a block's start and used after its data loads are
-// issued travel as raw vectors: a complex local held across the loads was
-// kept by the compiler in an LDS slot of its own (16 KiB more LDS and a wait
-// for the read before the first data load).
-template <typename R>
-struct Ct3Raw {
-    typedef tope_u32x4 type;
-};
-template <>
-struct Ct3Raw<float> {
-    typedef tope_u32x2 type;
-};
-template <typename R>
-__device__ __forceinline__ typename Ct3Raw<R>::type raw_ld(const cpx<R> *p)
-{
-    return *reinterpret_cast<const typename Ct3Raw<R>::type *>(p);
-}
-template <typename R>
-__device__ __forceinline__ cpx<R> raw_cpx(typename Ct3Raw<R>::type u)
-{
-    return __builtin_bit_cast(cpx<R>, u);
-}
-
 // W_{pt_n}^x as the two table values whose product it is (pt_lookup's
 // operands): the reads are issued here, the product formed later.
 template <typename R>
-__device__ __forceinline__ void pt_fetch(long long x, const AxisArgs<R> &a, typename Ct3Raw<R>::type &lo,
-                                         typename Ct3Raw<R>::type &hi)
+__device__ __forceinline__ void pt_fetch(long long x, const AxisArgs<R> &a, typename RawCpx<R>::type &lo,
+                                         typename RawCpx<R>::type &hi)
 {
     x &= a.pt_n - 1;
     lo = raw_ld(a.pt_lo + (x & ((1LL << a.pt_shift) - 1)));
@@ -112,7 +89,7 @@ __global__ void __launch_bounds__(1024) ct3_kernel(const cpx<R> *in, cpx<R> *out
     // out (2D 4096^2 Y pass 105.0-106.3 -> 103.6-105.4 us).  With it the
     // tile is at its register budget, and the held value made the inverse
     // instantiation spill (1D 2^24 307 -> 314 us): stored at once there.
-    typename Ct3Raw<R>::type tabv{};
+    typename RawCpx<R>::type tabv{};
     if constexpr (POST) {
         if (tid < CT3_TW) ltw[tid] = a.tw[tid];
     } else {
@@ -337,7 +314,7 @@ __global__ void __launch_bounds__(1024) ct3_pair_kernel(const cpx<float> *in, cp
     const int tid = threadIdx.x;
     const int t = tid & 3, g = (tid >> 2) & 15, w = tid >> 6;
     // step C's twiddle rows: read now, stored to LDS once the data loads are issued
-    const typename Ct3Raw<R>::type tabv = raw_ld(a.tw + (tid < CT3_TW ? tid : 0));  // as in ct3_kernel
+    const typename RawCpx<R>::type tabv = raw_ld(a.tw + (tid < CT3_TW ? tid : 0));  // as in ct3_kernel
 
     const long long blk = xcd_block(a.xcd);
     const long long o = blk / a.tiles;
@@ -349,7 +326,7 @@ __global__ void __launch_bounds__(1024) ct3_pair_kernel(const cpx<float> *in, cp
     // the post-twiddle factors of the tile's columns: read now, formed and
     // stored to LDS once the data loads are issued
     cpx<R> *ptl = ltw + CT3_TW;
-    typename Ct3Raw<R>::type plo{}, phi{};
+    typename RawCpx<R>::type plo{}, phi{};
     if constexpr (POST) {
         const int kind = tid >> 7, cc = (tid >> 4) & 7, j = tid & 15;
         const long long bt = o * a.pt_o + (long long)(c0 + cc) * a.pt_c;

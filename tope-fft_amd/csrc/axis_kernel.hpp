@@ -192,6 +192,29 @@ __device__ __forceinline__ void buf_store(cpx<R> v, __amdgpu_buffer_rsrc_t r, in
         __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(tope_u32x2, v), r, voff, soff, 0);
 }
 
+// Table values read at a block's start and used after its data loads are
+// issued travel as raw vectors: a complex local held across the loads was
+// kept by the compiler in an LDS slot of its own (16 KiB more LDS and a wait
+// for the read before the first data load).
+template <typename R>
+struct RawCpx {
+    typedef tope_u32x4 type;
+};
+template <>
+struct RawCpx<float> {
+    typedef tope_u32x2 type;
+};
+template <typename R>
+__device__ __forceinline__ typename RawCpx<R>::type raw_ld(const cpx<R> *p)
+{
+    return *reinterpret_cast<const typename RawCpx<R>::type *>(p);
+}
+template <typename R>
+__device__ __forceinline__ cpx<R> raw_cpx(typename RawCpx<R>::type u)
+{
+    return __builtin_bit_cast(cpx<R>, u);
+}
+
 // W_{pt_n}^x = pt_lo[x & mask] * pt_hi[x >> pt_shift] (x reduced mod pt_n).
 template <typename R>
 __device__ __forceinline__ cpx<R> pt_lookup(long long x, const cpx<R> *pt_lo, const cpx<R> *pt_hi, long long pt_n,
@@ -440,11 +463,20 @@ __device__ __forceinline__ void axis_block(const cpx<R> *in, cpx<R> *out, const 
     const int tid = threadIdx.x;
     const int t = CFAST ? (tid % TB) : (tid / P);
     const int p = CFAST ? (tid / TB) : (tid % P);
-    // LDS twiddle rows, loaded ahead of the data; stage 1 reads them after
-    // the first exchange's barriers
+    // LDS twiddle rows (stage 1 reads them after the first exchange's
+    // barriers): read into registers ahead of the data loads, stored once
+    // they are issued -- a store right after each read waited for it before
+    // the first data load went out
     cpx<R> *twl = reinterpret_cast<cpx<R> *>(lds + XB);
+    constexpr int NTH = TB * P;
+    constexpr int TWN = TWL ? (TwLds<N, E>::TOTAL + NTH - 1) / NTH : 1;
+    typename RawCpx<R>::type twr[TWN];
     if constexpr (TWL) {
-        for (int i = tid; i < TwLds<N, E>::TOTAL; i += TB * P) twl[i] = a.tw[TwLds<N, E>::src(i)];
+#pragma unroll
+        for (int j = 0; j < TWN; ++j) {
+            const int i = tid + j * NTH;
+            twr[j] = raw_ld(a.tw + TwLds<N, E>::src(i < TwLds<N, E>::TOTAL ? i : 0));
+        }
     }
     const long long o = blk / a.tiles;
     int tile = (int)(blk - o * a.tiles) + a.rot;
@@ -482,6 +514,12 @@ __device__ __forceinline__ void axis_block(const cpx<R> *in, cpx<R> *out, const 
     }
 
     if constexpr (TWL) {
+        asm volatile("" ::: "memory");  // the stores stay after the data loads
+#pragma unroll
+        for (int j = 0; j < TWN; ++j) {
+            const int i = tid + j * NTH;
+            if (i < TwLds<N, E>::TOTAL) twl[i] = raw_cpx<R>(twr[j]);
+        }
         run_stages<R, N, E, TB, CFAST, SPLIT, SIGN, 0, true>(v, p, t, a.tw, lds, twl);
     } else {
         run_stages<R, N, E, TB, CFAST, SPLIT, SIGN, 0>(v, p, t, a.tw, lds);
