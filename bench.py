@@ -433,8 +433,8 @@ def main():
                     help="comma list of mp2d,1d,long,3d256,mp,many,3d512,r2c,vendor,yard (default all; 2d is always run)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(each box has one MI355X); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
+              "(one process, one MI355X); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import topefft
